@@ -1,0 +1,166 @@
+/*
+ * bacchus_hip.h - C ABI of the MI355X (gfx950) Bacchus hot path: the
+ * index-list driven block Hamiltonian x vector product (Minerva) and the
+ * Lanczos recurrence around it.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * nonzero code on failure; bacchus_hip_last_error() then holds the message
+ * ("Error in <function> @ <file>:<line>: ..." like the reference's error()
+ * macro, src/Utilities/error/error.h:7-13).  The file-based entry points
+ * keep the reference's inputs unchanged: comb.txt, the BLOCK: evaluation
+ * order, index_list_<id>, <id>_matrix_elements and vec_<id> directories.
+ */
+#ifndef BACCHUS_HIP_H
+#define BACCHUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Storage precision of the matrix elements on the device. */
+enum bacchus_hip_precision {
+	BACCHUS_HIP_FP64 = 0, /* FP64 elements, FP64 accumulation            */
+	BACCHUS_HIP_FP32 = 1, /* FP32 elements, FP32 products and accumulation */
+	BACCHUS_HIP_BF16 = 2  /* BF16 elements, FP32 products and accumulation */
+};
+
+/* One evaluation-order instruction with dense 0-based indices
+ * (src/Minerva/evaluation_order/evaluation_order.h:22-31).
+ * type: 1 = neutron, 2 = proton, 3 = neutron-proton block
+ * (evaluation_order.h:13-20); -1 marks an unused list. */
+typedef struct bacchus_hip_instruction {
+	int32_t type;
+	int32_t num_particles;
+	int32_t block_in;
+	int32_t block_out;
+	int32_t neutron_list;
+	int32_t proton_list;
+	int32_t matrix_block;
+	int32_t pad;
+} bacchus_hip_instruction_t;
+
+/* Host-side description of an operator, the in-memory form of the files a
+ * scheduler reads (src/Minerva/memory_manager/memory_manager.c:455-509). */
+typedef struct bacchus_hip_problem {
+	int64_t num_blocks;
+	const int64_t *block_np;         /* proton states per basis block   */
+	const int64_t *block_nn;         /* neutron states per basis block  */
+	int64_t num_lists;
+	const int64_t *list_offsets;     /* num_lists+1, in triples         */
+	const int32_t *list_entries;     /* {in, out, matrix} triples       */
+	int64_t num_matrix_blocks;
+	const int64_t *matrix_offsets;   /* num_matrix_blocks+1, elements   */
+	const int64_t *matrix_dn;        /* neutron matrix dimension        */
+	const int32_t *matrix_force;     /* 1, 2 or 3 body (scale class)    */
+	const double *matrix_elements;
+	int64_t num_instructions;
+	const bacchus_hip_instruction_t *instructions;
+} bacchus_hip_problem_t;
+
+typedef struct bacchus_hip_operator bacchus_hip_operator_t;
+
+/* Validate the problem on the host (every index in bounds), build the
+ * device tables and upload everything to HBM on the current device. */
+int bacchus_hip_operator_create(const bacchus_hip_problem_t *problem,
+				int precision, bacchus_hip_operator_t **op);
+
+/* File-based constructor: replaces new_combination_table
+ * (src/Utilities/combination_table/combination_table.c:83-137) +
+ * read_evaluation_order (src/Minerva/evaluation_order/evaluation_order.c:
+ * 43-112) + new_scheduler (src/Minerva/scheduler/scheduler.c:62-78).
+ * Reads comb.txt, the evaluation order, index_list_<id> and
+ * <id>_matrix_elements files and uploads them once. */
+int bacchus_hip_operator_load(const char *combination_table_file,
+			      int num_protons, int num_neutrons,
+			      const char *evaluation_order_file,
+			      const char *index_lists_base_directory,
+			      const char *matrix_file_base_directory,
+			      int precision, bacchus_hip_operator_t **op);
+
+void bacchus_hip_operator_free(bacchus_hip_operator_t *op);
+
+int64_t bacchus_hip_operator_dimension(const bacchus_hip_operator_t *op);
+
+/* Scale factors per force class (1-, 2-, 3-body); default 1, 1, 1.
+ * H(c1, c2) = H1 + c1*V2 + c2*V3 without rewriting element blocks
+ * (the linear combination src/Vulcan/programs/vulcan.c:114-207 writes). */
+int bacchus_hip_operator_set_scales(bacchus_hip_operator_t *op, double s1,
+				    double s2, double s3);
+
+/* y = H x (accumulate = 0) or y += H x (accumulate = 1) on device memory,
+ * D doubles each, on the given hipStream_t (NULL = default stream).
+ * Device-resident form of run_matrix_vector_multiplication
+ * (src/Minerva/scheduler/scheduler.c:80-145). */
+int bacchus_hip_operator_matvec(bacchus_hip_operator_t *op, const double *x,
+				double *y, int accumulate, void *stream);
+
+/* Algorithmic traffic and work of one matvec: bytes = s_V * sum|V| +
+ * 12 * sum|L| + 16 * D, flops = 2 * multiplications (off-diagonal
+ * instructions count twice). */
+int bacchus_hip_operator_stats(const bacchus_hip_operator_t *op,
+			       double *algorithmic_bytes, double *flops,
+			       int64_t *num_work_items);
+
+/* Drop-in for run_matrix_vector_multiplication
+ * (src/Minerva/scheduler/scheduler.c:80-145): reads <in>/vec_<id>, adds
+ * H x to the existing <out>/vec_<id> files (the reference loads the output
+ * files and accumulates, src/Minerva/vector_block/vector_block.c:274-294)
+ * and writes them back. */
+int bacchus_hip_run_matrix_vector_multiplication(
+	const char *output_vector_base_directory,
+	const char *input_vector_base_directory, bacchus_hip_operator_t *op);
+
+/* Lanczos on the device (src/Bacchus/lanczos/lanczos.c:60-253). */
+enum bacchus_hip_start { BACCHUS_HIP_START_E0 = 0,     /* lanczos.c:60-72 */
+			 BACCHUS_HIP_START_RANDOM = 1, /* seeded, normalized */
+			 BACCHUS_HIP_START_GIVEN = 2 };
+enum bacchus_hip_reorth { BACCHUS_HIP_REORTH_REFERENCE = 0, /* MGS vs
+							      u_0..u_{k-3},
+							      lanczos.c:155-169 */
+			  BACCHUS_HIP_REORTH_FULL = 1,     /* CGS2 vs all */
+			  BACCHUS_HIP_REORTH_NONE = 2 };
+enum bacchus_hip_convergence { BACCHUS_HIP_CONVERGE_EIGENVALUES = 0,
+			       BACCHUS_HIP_CONVERGE_NONE = 2,
+			       BACCHUS_HIP_CONVERGE_TARGETS = 3 };
+
+typedef struct bacchus_hip_lanczos_settings {
+	int64_t max_num_iterations;
+	int32_t start;            /* enum bacchus_hip_start            */
+	int32_t reorthogonalization; /* enum bacchus_hip_reorth         */
+	int32_t convergence;      /* enum bacchus_hip_convergence      */
+	int32_t num_eigenvalues;  /* lowest eigenvalues reported       */
+	double eigenvalue_tolerance;
+	uint64_t seed;            /* random start                       */
+	const double *start_vector; /* device pointer, START_GIVEN     */
+	const double *target_eigenvalues; /* CONVERGE_TARGETS: stop when the
+					     lowest num_eigenvalues are within
+					     eigenvalue_tolerance (relative) */
+	int32_t check_every;      /* iterations between eigenvalue checks */
+	int32_t pad;
+} bacchus_hip_lanczos_settings_t;
+
+typedef struct bacchus_hip_lanczos_result {
+	int64_t iterations;       /* Krylov dimension reached            */
+	double matvec_seconds;    /* device time in the matvec            */
+	double total_seconds;     /* wall time of the solve               */
+} bacchus_hip_lanczos_result_t;
+
+/* Runs Lanczos; writes the lowest num_eigenvalues Ritz values to
+ * eigenvalues (host), alpha/beta (host, max_num_iterations each, may be
+ * NULL), and, if ritz_vector is non-NULL (device, D doubles), the Ritz
+ * vector of the lowest eigenvalue. */
+int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
+			const bacchus_hip_lanczos_settings_t *settings,
+			double *eigenvalues, double *alpha, double *beta,
+			double *ritz_vector,
+			bacchus_hip_lanczos_result_t *result);
+
+const char *bacchus_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,70 @@
+"""Build the native libraries in-tree.
+
+* ``jupiterncsm_amd/libbacchus_hip.so`` - the gfx950 HIP kernels and the C ABI
+  declared in ``include/bacchus_hip.h``;
+* ``jupiterncsm_amd/libncsm_inputs.so`` - the synthetic input generator
+  declared in ``include/ncsm_inputs.h``.
+
+Run ``python -m jupiterncsm_amd.build`` or ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["operator.hip", "files.hip", "lanczos.hip"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions stay off on this target
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP64 atomic add as the hardware instruction, not a CAS loop
+    "-munsafe-fp-atomics",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+    return proc
+
+
+def build_inputs(verbose=False):
+    out = os.path.join(HERE, "libncsm_inputs.so")
+    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+          f"-I{INCLUDE}", os.path.join(CSRC, "ncsm_inputs.cpp"), "-o", out], verbose)
+    return out
+
+
+def build_hip(verbose=False):
+    out = os.path.join(HERE, "libbacchus_hip.so")
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, "_build", src.replace(".hip", ".o"))
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+              os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out], verbose)
+    return out
+
+
+def build_all(verbose=False):
+    return [build_inputs(verbose), build_hip(verbose)]
+
+
+if __name__ == "__main__":
+    for path in build_all(verbose=True):
+        print("built", path)

@@ -1,0 +1,364 @@
+// files.hip - the file-based boundary: comb.txt, the BLOCK: evaluation
+// order, index_list_<id>, <id>_matrix_elements and vec_<id> directories,
+// read and written with the reference's layouts.
+#include "bacchus_internal.h"
+
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace bacchus {
+namespace {
+
+std::vector<std::string> split_words(const std::string &row)
+{
+	// whitespace = ' ', '\n', '\t' (src/Utilities/string_tools/
+	// string_tools.c:222-226)
+	std::vector<std::string> words;
+	size_t i = 0;
+	while (i < row.size()) {
+		while (i < row.size() && (row[i] == ' ' || row[i] == '\n' || row[i] == '\t'))
+			i++;
+		size_t j = i;
+		while (j < row.size() && !(row[j] == ' ' || row[j] == '\n' || row[j] == '\t'))
+			j++;
+		if (j > i)
+			words.push_back(row.substr(i, j - i));
+		i = j;
+	}
+	return words;
+}
+
+// digits up to '=' (combination_table.c:715-723)
+int64_t id_of(const std::string &w)
+{
+	int64_t id = 0;
+	for (char c : w) {
+		if (c == '=')
+			break;
+		id = 10 * id + (c - '0');
+	}
+	return id;
+}
+
+bool read_lines(const char *path, std::vector<std::string> &lines)
+{
+	FILE *f = fopen(path, "r");
+	if (!f)
+		return false;
+	char *row = nullptr;
+	size_t cap = 0;
+	ssize_t n;
+	while ((n = getline(&row, &cap, f)) >= 0)
+		lines.emplace_back(row, (size_t)n);
+	free(row);
+	fclose(f);
+	return true;
+}
+
+struct CombTable {
+	std::vector<int64_t> block_ids, block_np, block_nn;
+	std::map<int64_t, int64_t> block_index;  // id -> dense
+	std::map<int64_t, char> list_type;       // id -> 'n' / 'p'
+	std::map<int64_t, int> matrix_rank;      // id -> 1, 2, 3 body
+};
+
+// Sections by title (combination_table.c:419-640): rows of a section run
+// from the first row containing the title to the next row containing "***".
+int parse_comb(const char *path, CombTable &t)
+{
+	std::vector<std::string> lines;
+	if (!read_lines(path, lines))
+		return BACCHUS_FAIL("Could not open \"%s\". %s", path, strerror(errno));
+	auto section = [&](const char *title, auto &&fn) {
+		size_t i = 0;
+		while (i < lines.size() && lines[i].find(title) == std::string::npos)
+			i++;
+		for (i++; i < lines.size(); i++) {
+			if (lines[i].find("***") != std::string::npos)
+				break;
+			fn(lines[i]);
+		}
+	};
+	int bad = 0;
+	section("mp-states", [&](const std::string &row) {
+		if (row.find("ARRAYMP:") == std::string::npos)
+			return;
+		auto w = split_words(row);
+		if (w.size() < 10) {
+			bad = 1;
+			return;
+		}
+		const int64_t id = id_of(w[9]);
+		t.block_index[id] = (int64_t)t.block_ids.size();
+		t.block_ids.push_back(id);
+		t.block_np.push_back(atoll(w[4].c_str()));
+		t.block_nn.push_back(atoll(w[5].c_str()));
+	});
+	section("Conn lists", [&](const std::string &row) {
+		if (row.find("ARRAY:") == std::string::npos)
+			return;
+		auto w = split_words(row);
+		if (w.size() < 10 || (w[0][0] != 'n' && w[0][0] != 'p')) {
+			bad = 1;
+			return;
+		}
+		t.list_type[id_of(w[9])] = w[0][0];
+	});
+	// cross p-n rows: type = word 4 + word 0, id = word 13; same p/n rows:
+	// type = word 0, id = word 8 (combination_table.c:535-637)
+	section("Matrix-elements V (cross p-n)", [&](const std::string &row) {
+		if (row.find("ARRAY:") == std::string::npos)
+			return;
+		auto w = split_words(row);
+		if (w.size() < 14) {
+			bad = 1;
+			return;
+		}
+		t.matrix_rank[id_of(w[13])] = (int)(w[0].size() + w[4].size());
+	});
+	section("Matrix-elements V (same p/n)", [&](const std::string &row) {
+		if (row.find("ARRAY:") == std::string::npos)
+			return;
+		auto w = split_words(row);
+		if (w.size() < 9) {
+			bad = 1;
+			return;
+		}
+		t.matrix_rank[id_of(w[8])] = (int)w[0].size();
+	});
+	if (bad)
+		return BACCHUS_FAIL("malformed ARRAYMP/ARRAY row in %s", path);
+	if (t.block_ids.empty())
+		return BACCHUS_FAIL("no basis blocks (ARRAYMP rows) in %s", path);
+	return 0;
+}
+
+int read_file(const std::string &path, std::vector<char> &buf)
+{
+	FILE *f = fopen(path.c_str(), "rb");
+	if (!f)
+		return BACCHUS_FAIL("Could not open file %s. %s", path.c_str(), strerror(errno));
+	fseek(f, 0, SEEK_END);
+	const long n = ftell(f);
+	fseek(f, 0, SEEK_SET);
+	buf.resize(n > 0 ? (size_t)n : 0);
+	if (n > 0 && fread(buf.data(), 1, (size_t)n, f) != (size_t)n) {
+		fclose(f);
+		return BACCHUS_FAIL("Could not read %s", path.c_str());
+	}
+	fclose(f);
+	return 0;
+}
+
+}  // namespace
+}  // namespace bacchus
+
+using namespace bacchus;
+
+extern "C" {
+
+int bacchus_hip_operator_load(const char *combination_table_file,
+			      int num_protons, int num_neutrons,
+			      const char *evaluation_order_file,
+			      const char *index_lists_base_directory,
+			      const char *matrix_file_base_directory,
+			      int precision, bacchus_hip_operator_t **op)
+{
+	(void)num_protons;
+	(void)num_neutrons;
+	if (!combination_table_file || !evaluation_order_file ||
+	    !index_lists_base_directory || !matrix_file_base_directory || !op)
+		return BACCHUS_FAIL("NULL argument");
+	CombTable comb;
+	if (parse_comb(combination_table_file, comb))
+		return 1;
+	// evaluation order (evaluation_order.c:175-243): rows with "BLOCK:",
+	// "UNLOAD_" rows ignored, 6 words = NP, else N or P by list type
+	std::vector<std::string> rows;
+	if (!read_lines(evaluation_order_file, rows))
+		return BACCHUS_FAIL("Could not open evaluation order file %s. %s",
+				    evaluation_order_file, strerror(errno));
+	std::vector<bacchus_hip_instruction_t> instr;
+	std::map<int64_t, int32_t> list_index, v_index;
+	std::vector<int64_t> list_ids, v_ids;
+	auto dense = [](std::map<int64_t, int32_t> &m, std::vector<int64_t> &ids, int64_t id) {
+		auto it = m.find(id);
+		if (it != m.end())
+			return it->second;
+		const int32_t k = (int32_t)ids.size();
+		ids.push_back(id);
+		m[id] = k;
+		return k;
+	};
+	for (const std::string &row : rows) {
+		const size_t at = row.find("BLOCK:");
+		if (at == std::string::npos || row.find("UNLOAD_") != std::string::npos)
+			continue;
+		auto w = split_words(row.substr(at));
+		if (w.size() != 5 && w.size() != 6)
+			return BACCHUS_FAIL("evaluation order row \"%s\" is not formatted "
+					    "correctly", row.c_str());
+		bacchus_hip_instruction_t I{};
+		const int64_t bin = atoll(w[1].c_str()), bout = atoll(w[2].c_str());
+		if (!comb.block_index.count(bin) || !comb.block_index.count(bout))
+			return BACCHUS_FAIL("evaluation order row \"%s\" names an unknown "
+					    "basis block", row.c_str());
+		I.block_in = (int32_t)comb.block_index[bin];
+		I.block_out = (int32_t)comb.block_index[bout];
+		auto type_of = [&](int64_t id) -> char {
+			auto it = comb.list_type.find(id);
+			return it == comb.list_type.end() ? 0 : it->second;
+		};
+		if (w.size() == 6) {
+			int64_t l1 = atoll(w[3].c_str()), l2 = atoll(w[4].c_str());
+			if (!type_of(l1) || !type_of(l2))
+				return BACCHUS_FAIL("row \"%s\" names an unknown index list",
+						    row.c_str());
+			if (type_of(l1) != 'n')
+				std::swap(l1, l2);
+			I.type = 3;
+			I.neutron_list = dense(list_index, list_ids, l1);
+			I.proton_list = dense(list_index, list_ids, l2);
+			I.matrix_block = dense(v_index, v_ids, atoll(w[5].c_str()));
+		} else {
+			const int64_t l = atoll(w[3].c_str());
+			const char c = type_of(l);
+			if (!c)
+				return BACCHUS_FAIL("row \"%s\" names an unknown index list",
+						    row.c_str());
+			I.type = c == 'n' ? 1 : 2;
+			I.neutron_list = c == 'n' ? dense(list_index, list_ids, l) : -1;
+			I.proton_list = c == 'n' ? -1 : dense(list_index, list_ids, l);
+			I.matrix_block = dense(v_index, v_ids, atoll(w[4].c_str()));
+		}
+		instr.push_back(I);
+	}
+	// index lists (index_list.c:69-97)
+	std::vector<int64_t> list_offsets{0};
+	std::vector<int32_t> list_entries;
+	std::vector<char> buf;
+	for (int64_t id : list_ids) {
+		char name[64];
+		snprintf(name, sizeof name, "/index_list_%lld", (long long)id);
+		if (read_file(std::string(index_lists_base_directory) + name, buf))
+			return 1;
+		if (buf.size() % 12 != 0)
+			return BACCHUS_FAIL("index list %lld is not a whole number of "
+					    "triples", (long long)id);
+		const size_t n = buf.size() / 4;
+		const size_t old = list_entries.size();
+		list_entries.resize(old + n);
+		if (n)
+			memcpy(&list_entries[old], buf.data(), buf.size());
+		list_offsets.push_back((int64_t)list_entries.size() / 3);
+	}
+	// matrix elements (matrix_block.c:22-62): size_t Dn, size_t Dp,
+	// double[n] with n = Dn*Dp, or Dp if Dn = 0, or Dn if Dp = 0
+	std::vector<int64_t> v_offsets{0}, v_dn;
+	std::vector<int32_t> v_force;
+	std::vector<double> elements;
+	for (int64_t id : v_ids) {
+		char name[64];
+		snprintf(name, sizeof name, "/%lld_matrix_elements", (long long)id);
+		if (read_file(std::string(matrix_file_base_directory) + name, buf))
+			return 1;
+		if (buf.size() < 16)
+			return BACCHUS_FAIL("Could not read neutron_matrix_dimension from "
+					    "matrix file %lld", (long long)id);
+		uint64_t dn, dp;
+		memcpy(&dn, buf.data(), 8);
+		memcpy(&dp, buf.data() + 8, 8);
+		const uint64_t n = dn == 0 ? dp : (dp == 0 ? dn : dn * dp);
+		if (buf.size() < 16 + 8 * n)
+			return BACCHUS_FAIL("Could not read matrix elements form matrix "
+					    "file %lld", (long long)id);
+		const size_t old = elements.size();
+		elements.resize(old + n);
+		if (n)
+			memcpy(&elements[old], buf.data() + 16, 8 * n);
+		v_offsets.push_back((int64_t)elements.size());
+		v_dn.push_back((int64_t)dn);
+		auto r = comb.matrix_rank.find(id);
+		v_force.push_back(r == comb.matrix_rank.end() ? 2 : std::min(3, std::max(1, r->second)));
+	}
+	bacchus_hip_problem_t p{};
+	p.num_blocks = (int64_t)comb.block_ids.size();
+	p.block_np = comb.block_np.data();
+	p.block_nn = comb.block_nn.data();
+	p.num_lists = (int64_t)list_ids.size();
+	p.list_offsets = list_offsets.data();
+	p.list_entries = list_entries.data();
+	p.num_matrix_blocks = (int64_t)v_ids.size();
+	p.matrix_offsets = v_offsets.data();
+	p.matrix_dn = v_dn.data();
+	p.matrix_force = v_force.data();
+	p.matrix_elements = elements.data();
+	p.num_instructions = (int64_t)instr.size();
+	p.instructions = instr.data();
+	const int rc = bacchus_hip_operator_create(&p, precision, op);
+	if (rc)
+		return rc;
+	(*op)->block_ids = comb.block_ids;
+	return 0;
+}
+
+int bacchus_hip_run_matrix_vector_multiplication(
+	const char *output_vector_base_directory,
+	const char *input_vector_base_directory, bacchus_hip_operator_t *op)
+{
+	if (!op || !output_vector_base_directory || !input_vector_base_directory)
+		return BACCHUS_FAIL("NULL argument");
+	const int64_t nb = (int64_t)op->block_np.size();
+	const std::vector<int64_t> &ids = op->block_ids;
+	const int64_t D = op->dimension;
+	std::vector<double> x(D), y(D);
+	auto io = [&](const char *dir, std::vector<double> &v, bool write) -> int {
+		for (int64_t b = 0; b < nb; b++) {
+			char name[64];
+			snprintf(name, sizeof name, "/vec_%lld", (long long)ids[b]);
+			const std::string path = std::string(dir) + name;
+			FILE *f = fopen(path.c_str(), write ? "wb" : "rb");
+			if (!f)
+				return BACCHUS_FAIL("Could not open vector block file %s. %s",
+						    path.c_str(), strerror(errno));
+			const size_t n = (size_t)(op->block_off[b + 1] - op->block_off[b]);
+			const size_t done = write ? fwrite(&v[op->block_off[b]], 8, n, f)
+						  : fread(&v[op->block_off[b]], 8, n, f);
+			fclose(f);
+			if (done != n)
+				return BACCHUS_FAIL("Could not %s elements of block %lld in %s",
+						    write ? "write" : "read", (long long)ids[b], dir);
+		}
+		return 0;
+	};
+	if (io(input_vector_base_directory, x, false) ||
+	    io(output_vector_base_directory, y, false))
+		return 1;
+	double *dx = nullptr, *dy = nullptr;
+	BACCHUS_HIP_CHECK(hipMalloc(&dx, D * 8));
+	if (hipMalloc(&dy, D * 8) != hipSuccess) {
+		(void)hipFree(dx);
+		return BACCHUS_FAIL("hipMalloc of %lld bytes failed", (long long)(D * 8));
+	}
+	int rc = 0;
+	if (hipMemcpy(dx, x.data(), D * 8, hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(dy, y.data(), D * 8, hipMemcpyHostToDevice) != hipSuccess)
+		rc = BACCHUS_FAIL("host to device copy failed");
+	if (!rc)
+		rc = bacchus_hip_operator_matvec(op, dx, dy, 1, nullptr);
+	if (!rc && hipMemcpy(y.data(), dy, D * 8, hipMemcpyDeviceToHost) != hipSuccess)
+		rc = BACCHUS_FAIL("device to host copy failed");
+	(void)hipFree(dx);
+	(void)hipFree(dy);
+	if (rc)
+		return rc;
+	return io(output_vector_base_directory, y, true);
+}
+
+}  // extern "C"
