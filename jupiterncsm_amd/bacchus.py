@@ -1,0 +1,298 @@
+"""Host-side mirror of the reference's Bacchus/Minerva interface over the
+gfx950 library ``libbacchus_hip.so`` (C ABI in ``include/bacchus_hip.h``).
+
+Reference interface                                   here
+----------------------------------------------------  ---------------------------------
+new_generative_matrix / new_scheduler                 :meth:`Operator.load`,
+  (src/Bacchus/matrix/matrix.c:52-66,                  :meth:`Operator.from_inputs`
+   src/Minerva/scheduler/scheduler.c:62-78)
+matrix_vector_multiplication                          :func:`matrix_vector_multiplication`
+  (src/Bacchus/matrix/matrix.c:68-96)
+run_matrix_vector_multiplication                      :func:`run_matrix_vector_multiplication`
+  (src/Minerva/scheduler/scheduler.c:80-145)
+free_scheduler / free_matrix                          :meth:`Operator.close`
+new_lanczos_environment + diagonalize +               :func:`diagonalize`
+  get_eigensystem (src/Bacchus/lanczos/lanczos.c:
+  75-96,209-303)
+
+Errors surface as :class:`BacchusError` carrying the library's
+"Error in <func> @ <file>:<line>" message (the reference prints the same
+shape and exits, src/Utilities/error/error.h:7-13).  There is no CPU
+fallback: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FP64, FP32, BF16 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16}
+START_E0, START_RANDOM, START_GIVEN = 0, 1, 2
+REORTH_REFERENCE, REORTH_FULL, REORTH_NONE = 0, 1, 2
+CONVERGE_EIGENVALUES, CONVERGE_NONE, CONVERGE_TARGETS = 0, 2, 3
+
+EXPORTED_SYMBOLS = (
+    "bacchus_hip_operator_create", "bacchus_hip_operator_load", "bacchus_hip_operator_free",
+    "bacchus_hip_operator_dimension", "bacchus_hip_operator_set_scales",
+    "bacchus_hip_operator_matvec", "bacchus_hip_operator_stats",
+    "bacchus_hip_run_matrix_vector_multiplication", "bacchus_hip_lanczos",
+    "bacchus_hip_last_error",
+)
+
+
+class BacchusError(RuntimeError):
+    pass
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("num_blocks", ctypes.c_int64), ("block_np", ctypes.c_void_p),
+        ("block_nn", ctypes.c_void_p), ("num_lists", ctypes.c_int64),
+        ("list_offsets", ctypes.c_void_p), ("list_entries", ctypes.c_void_p),
+        ("num_matrix_blocks", ctypes.c_int64), ("matrix_offsets", ctypes.c_void_p),
+        ("matrix_dn", ctypes.c_void_p), ("matrix_force", ctypes.c_void_p),
+        ("matrix_elements", ctypes.c_void_p), ("num_instructions", ctypes.c_int64),
+        ("instructions", ctypes.c_void_p),
+    ]
+
+
+class _LanczosSettings(ctypes.Structure):
+    _fields_ = [
+        ("max_num_iterations", ctypes.c_int64), ("start", ctypes.c_int32),
+        ("reorthogonalization", ctypes.c_int32), ("convergence", ctypes.c_int32),
+        ("num_eigenvalues", ctypes.c_int32), ("eigenvalue_tolerance", ctypes.c_double),
+        ("seed", ctypes.c_uint64), ("start_vector", ctypes.c_void_p),
+        ("target_eigenvalues", ctypes.c_void_p), ("check_every", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class _LanczosResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("matvec_seconds", ctypes.c_double),
+                ("total_seconds", ctypes.c_double)]
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libbacchus_hip.so")
+
+
+def library() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise BacchusError(f"{path} is missing: build it with __graft_entry__.build()")
+    # torch bundles its own HIP runtime under the same SONAME
+    # (libamdhip64.so.7); loading it first makes this library bind to that
+    # one, so device pointers and streams are shared by a single runtime.
+    import torch  # noqa: F401
+    lib = ctypes.CDLL(path)
+    vp, i64, c_int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+    lib.bacchus_hip_operator_create.restype = c_int
+    lib.bacchus_hip_operator_create.argtypes = [ctypes.POINTER(_Problem), c_int,
+                                                ctypes.POINTER(vp)]
+    lib.bacchus_hip_operator_load.restype = c_int
+    lib.bacchus_hip_operator_load.argtypes = [ctypes.c_char_p, c_int, c_int, ctypes.c_char_p,
+                                              ctypes.c_char_p, ctypes.c_char_p, c_int,
+                                              ctypes.POINTER(vp)]
+    lib.bacchus_hip_operator_free.restype = None
+    lib.bacchus_hip_operator_free.argtypes = [vp]
+    lib.bacchus_hip_operator_dimension.restype = i64
+    lib.bacchus_hip_operator_dimension.argtypes = [vp]
+    lib.bacchus_hip_operator_set_scales.restype = c_int
+    lib.bacchus_hip_operator_set_scales.argtypes = [vp, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double]
+    lib.bacchus_hip_operator_matvec.restype = c_int
+    lib.bacchus_hip_operator_matvec.argtypes = [vp, vp, vp, c_int, vp]
+    lib.bacchus_hip_operator_stats.restype = c_int
+    lib.bacchus_hip_operator_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(i64)]
+    lib.bacchus_hip_run_matrix_vector_multiplication.restype = c_int
+    lib.bacchus_hip_run_matrix_vector_multiplication.argtypes = [ctypes.c_char_p,
+                                                                 ctypes.c_char_p, vp]
+    lib.bacchus_hip_lanczos.restype = c_int
+    lib.bacchus_hip_lanczos.argtypes = [vp, ctypes.POINTER(_LanczosSettings), vp, vp, vp, vp,
+                                        ctypes.POINTER(_LanczosResult)]
+    lib.bacchus_hip_last_error.restype = ctypes.c_char_p
+    lib.bacchus_hip_last_error.argtypes = []
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise BacchusError(library().bacchus_hip_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Operator:
+    """Device-resident Hamiltonian: index lists, element blocks and the
+    instruction tables live in HBM from construction until :meth:`close`."""
+
+    def __init__(self, handle: ctypes.c_void_p, precision: int):
+        self._h = handle
+        self.precision = precision
+        self.dimension = int(library().bacchus_hip_operator_dimension(handle))
+
+    # -- construction ------------------------------------------------------
+    @classmethod
+    def from_arrays(cls, block_np, block_nn, list_offsets, list_entries, matrix_offsets,
+                    matrix_dn, matrix_force, matrix_elements, instructions, precision=FP64):
+        lib = library()
+        keep = [np.ascontiguousarray(block_np, dtype=np.int64),
+                np.ascontiguousarray(block_nn, dtype=np.int64),
+                np.ascontiguousarray(list_offsets, dtype=np.int64),
+                np.ascontiguousarray(list_entries, dtype=np.int32).reshape(-1),
+                np.ascontiguousarray(matrix_offsets, dtype=np.int64),
+                np.ascontiguousarray(matrix_dn, dtype=np.int64),
+                np.ascontiguousarray(matrix_force, dtype=np.int32),
+                np.ascontiguousarray(matrix_elements, dtype=np.float64),
+                np.ascontiguousarray(instructions, dtype=np.int32).reshape(-1, 8)]
+        p = _Problem(len(keep[0]), _ptr(keep[0]), _ptr(keep[1]), len(keep[2]) - 1,
+                     _ptr(keep[2]), _ptr(keep[3]), len(keep[4]) - 1, _ptr(keep[4]),
+                     _ptr(keep[5]), _ptr(keep[6]), _ptr(keep[7]), len(keep[8]), _ptr(keep[8]))
+        h = ctypes.c_void_p()
+        _check(lib.bacchus_hip_operator_create(ctypes.byref(p), precision, ctypes.byref(h)))
+        return cls(h, precision)
+
+    @classmethod
+    def from_inputs(cls, inputs, precision=FP64):
+        """Upload a generated system (:mod:`jupiterncsm_amd.inputs`)."""
+        ins = inputs.instructions
+        raw = np.stack([ins[f] for f in ins.dtype.names], axis=1).astype(np.int32)
+        return cls.from_arrays(inputs.blocks[:, 4], inputs.blocks[:, 5], inputs.list_offsets,
+                               inputs.list_entries, inputs.matrix_offsets,
+                               inputs.matrix_shapes[:, 0], inputs.matrix_shapes[:, 2],
+                               inputs.matrix_elements, raw, precision)
+
+    @classmethod
+    def load(cls, combination_table_file, num_protons, num_neutrons, evaluation_order_file,
+             index_lists_base_directory, matrix_file_base_directory, precision=FP64):
+        """From the reference's files (comb.txt, evaluation order,
+        index_list_<id>, <id>_matrix_elements)."""
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.bacchus_hip_operator_load(
+            str(combination_table_file).encode(), num_protons, num_neutrons,
+            str(evaluation_order_file).encode(), str(index_lists_base_directory).encode(),
+            str(matrix_file_base_directory).encode(), precision, ctypes.byref(h)))
+        return cls(h, precision)
+
+    # -- use ----------------------------------------------------------------
+    @property
+    def handle(self):
+        if self._h is None or not self._h.value:
+            raise BacchusError("operator is closed")
+        return self._h
+
+    def set_scales(self, s1=1.0, s2=1.0, s3=1.0):
+        """Scale the 1-, 2- and 3-body parts: H1*s1 + V2*s2 + V3*s3."""
+        _check(library().bacchus_hip_operator_set_scales(self.handle, s1, s2, s3))
+
+    def matvec(self, x, y, accumulate=False, stream=None):
+        """y = H x (or y += H x) on FP64 torch tensors on the GPU."""
+        import torch
+        if x.dtype != torch.float64 or y.dtype != torch.float64:
+            raise BacchusError("x and y must be float64")
+        if not (x.is_cuda and y.is_cuda and x.is_contiguous() and y.is_contiguous()):
+            raise BacchusError("x and y must be contiguous device tensors")
+        if x.numel() != self.dimension or y.numel() != self.dimension:
+            raise BacchusError(f"vectors must have {self.dimension} elements")
+        _check(library().bacchus_hip_operator_matvec(
+            self.handle, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+            int(bool(accumulate)), _stream_handle(stream)))
+        return y
+
+    def stats(self) -> dict:
+        b, f = ctypes.c_double(), ctypes.c_double()
+        w = ctypes.c_int64()
+        _check(library().bacchus_hip_operator_stats(self.handle, ctypes.byref(b),
+                                                    ctypes.byref(f), ctypes.byref(w)))
+        return {"algorithmic_bytes": b.value, "flops": f.value, "work_items": w.value}
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            library().bacchus_hip_operator_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def matrix_vector_multiplication(result, matrix: Operator, vector, stream=None):
+    """result = H vector (src/Bacchus/matrix/matrix.c:68-96)."""
+    return matrix.matvec(vector, result, accumulate=False, stream=stream)
+
+
+def run_matrix_vector_multiplication(output_vector_base_directory,
+                                     input_vector_base_directory, scheduler: Operator):
+    """Add H x to the vec_<id> files of the output directory, x read from
+    the input directory (src/Minerva/scheduler/scheduler.c:80-145)."""
+    _check(library().bacchus_hip_run_matrix_vector_multiplication(
+        str(output_vector_base_directory).encode(), str(input_vector_base_directory).encode(),
+        scheduler.handle))
+
+
+@dataclass
+class LanczosResult:
+    eigenvalues: np.ndarray
+    alpha: np.ndarray
+    beta: np.ndarray
+    iterations: int
+    matvec_seconds: float
+    total_seconds: float
+
+
+def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
+                start=START_E0, reorthogonalization=REORTH_REFERENCE,
+                convergence=CONVERGE_NONE, tolerance=1e-7, seed=1, start_vector=None,
+                target_eigenvalues=None, check_every=1, ritz_vector=None) -> LanczosResult:
+    """Lanczos on the device.  Defaults reproduce the reference
+    (e_0 start, MGS against u_0..u_{k-3}, lanczos.c:98-291)."""
+    lib = library()
+    targets = None
+    if target_eigenvalues is not None:
+        targets = np.ascontiguousarray(target_eigenvalues, dtype=np.float64)
+    st = _LanczosSettings(
+        int(max_num_iterations), start, reorthogonalization, convergence,
+        int(num_eigenvalues), float(tolerance), int(seed),
+        ctypes.c_void_p(start_vector.data_ptr()) if start_vector is not None else None,
+        _ptr(targets) if targets is not None else None, int(check_every), 0)
+    kmax = max(1, min(matrix.dimension, int(max_num_iterations)))
+    ev = np.zeros(max(1, num_eigenvalues))
+    alpha = np.zeros(kmax)
+    beta = np.zeros(kmax)
+    res = _LanczosResult()
+    rv = ctypes.c_void_p(ritz_vector.data_ptr()) if ritz_vector is not None else None
+    _check(lib.bacchus_hip_lanczos(matrix.handle, ctypes.byref(st), _ptr(ev), _ptr(alpha),
+                                   _ptr(beta), rv, ctypes.byref(res)))
+    k = int(res.iterations)
+    return LanczosResult(ev[:min(k, num_eigenvalues)], alpha[:k], beta[:k], k,
+                         res.matvec_seconds, res.total_seconds)

@@ -1,0 +1,114 @@
+"""Device Lanczos against the reference's known answers, the compiled
+reference's eigenvalues and dense diagonalization."""
+import os
+
+import numpy as np
+import pytest
+
+from jupiterncsm_amd import bacchus
+from jupiterncsm_amd.inputs import SystemConfig, build
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = {
+    "he4_nmax4": SystemConfig(2, 2, 4, max_body=2, seed=1),
+    "he4_nmax2_3b": SystemConfig(2, 2, 2, max_body=3, seed=5, sigma3=0.3),
+    "li6_nmax2_3b": SystemConfig(3, 3, 2, max_body=3, seed=11, sigma3=0.1),
+}
+
+
+def explicit_operator(M):
+    """A dense symmetric matrix as a one-block operator (the reference's
+    EXPLICIT matrix test backend, src/Bacchus/matrix/matrix.c:13-50): one
+    block of 1 x D states, one neutron list holding every (in, out) pair."""
+    D = M.shape[0]
+    ii, oo = np.meshgrid(np.arange(D), np.arange(D), indexing="ij")
+    entries = np.stack([ii.ravel(), oo.ravel(), (ii * D + oo).ravel()], axis=1)
+    V = np.array([M[o, i] for i, o in zip(ii.ravel(), oo.ravel())])
+    return bacchus.Operator.from_arrays([1], [D], [0, D * D], entries, [0, D * D], [D * D], [1],
+                                        V, np.array([[1, 1, 0, 0, 0, -1, 0, 0]]))
+
+
+def test_3x3_known_answer():
+    print("case lanczos 3x3", flush=True)
+    M = np.array([[1.0, 2, 3], [2, 1, 2], [3, 2, 1]])
+    with explicit_operator(M) as op:
+        r = bacchus.diagonalize(op, 3, num_eigenvalues=3)
+    assert r.iterations == 3
+    want = [-2.0, 2.5 - np.sqrt(41) / 2, 2.5 + np.sqrt(41) / 2]
+    assert np.allclose(r.eigenvalues, want, atol=1e-5)
+    assert np.allclose(r.eigenvalues, want, atol=1e-12)
+
+
+def test_50x50_random_against_dense():
+    print("case lanczos 50x50", flush=True)
+    rng = np.random.default_rng(50)
+    A = rng.uniform(-1, 1, (50, 50))
+    M = (A + A.T) / 2
+    with explicit_operator(M) as op:
+        r = bacchus.diagonalize(op, 50, num_eigenvalues=50,
+                                reorthogonalization=bacchus.REORTH_FULL)
+    assert np.abs(r.eigenvalues - np.linalg.eigvalsh(M)).max() < 1e-10
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_reference_mode_matches_compiled_reference(name, golden_dir):
+    """e_0 start, MGS against u_0..u_{k-3}, fixed iteration count, then the
+    convergence rule |lambda_0 - previous| < 1e-7: eigenvalues within 1e-8
+    relative of the compiled reference's (FP64)."""
+    print(f"case lanczos reference-mode {name}", flush=True)
+    g = np.load(os.path.join(golden_dir, f"{name}.npz"))
+    inp = build(CONFIGS[name])
+    iters = int(g["iterations_fixed"])
+    with bacchus.Operator.from_inputs(inp) as op:
+        r = bacchus.diagonalize(op, iters, num_eigenvalues=iters)
+        rc = bacchus.diagonalize(op, 200, num_eigenvalues=200,
+                                 convergence=bacchus.CONVERGE_EIGENVALUES, tolerance=1e-7)
+    ref = g["eig_ref_fixed"]
+    assert r.iterations == len(ref)
+    scale = np.abs(ref).max()
+    assert np.abs(r.eigenvalues - ref).max() <= 1e-8 * scale
+    assert rc.iterations == int(g["krylov_conv"])
+    assert np.abs(rc.eigenvalues - g["eig_ref_conv"]).max() <= 1e-8 * scale
+
+
+def test_full_reorthogonalization_random_start_against_dense(c1_inputs):
+    """The configs' Lanczos: seeded random start, full reorthogonalization,
+    lowest 5 eigenvalues within 1e-9 of dense diagonalization."""
+    print("case lanczos full c1", flush=True)
+    p = O.Problem.from_inputs(c1_inputs)
+    D = c1_inputs.dimension
+    H = np.zeros((D, D))
+    for i in range(D):
+        e = np.zeros(D)
+        e[i] = 1.0
+        H[:, i] = O.matvec(p, e)
+    w = np.linalg.eigvalsh(H)
+    import torch
+    ritz = torch.zeros(D, dtype=torch.float64, device="cuda")
+    with bacchus.Operator.from_inputs(c1_inputs) as op:
+        r = bacchus.diagonalize(op, 200, num_eigenvalues=5, start=bacchus.START_RANDOM,
+                                reorthogonalization=bacchus.REORTH_FULL, seed=7,
+                                ritz_vector=ritz)
+    assert r.iterations == 200
+    assert np.abs(r.eigenvalues - w[:5]).max() < 1e-9 * abs(w[0])
+    v = ritz.cpu().numpy()
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-8
+    assert np.linalg.norm(H @ v - w[0] * v) < 1e-6 * abs(w[0])
+
+
+def test_target_convergence_stops_early(c1_inputs):
+    print("case lanczos targets c1", flush=True)
+    p = O.Problem.from_inputs(c1_inputs)
+    ev, *_ = O.lanczos(c1_inputs.dimension, lambda v: O.matvec(p, v), 200,
+                       start=O.START_GIVEN,
+                       start_vector=np.ones(c1_inputs.dimension) / np.sqrt(c1_inputs.dimension),
+                       reorth=O.REORTH_FULL, num_eigenvalues=5)
+    with bacchus.Operator.from_inputs(c1_inputs) as op:
+        r = bacchus.diagonalize(op, 200, num_eigenvalues=5, start=bacchus.START_RANDOM,
+                                reorthogonalization=bacchus.REORTH_FULL, seed=3,
+                                convergence=bacchus.CONVERGE_TARGETS, tolerance=1e-6,
+                                target_eigenvalues=ev, check_every=5)
+    assert r.iterations < 200
+    assert np.all(np.abs(r.eigenvalues - ev) <= 1e-6 * np.abs(ev))

@@ -1,0 +1,154 @@
+"""Parity of the gfx950 matvec (through the C ABI) with the CPU oracle and
+the compiled reference's golden outputs."""
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from jupiterncsm_amd import bacchus
+from jupiterncsm_amd.inputs import SYSTEMS, SystemConfig, build
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = {
+    "he4_nmax4": SystemConfig(2, 2, 4, max_body=2, seed=1),
+    "he4_nmax2_3b": SystemConfig(2, 2, 2, max_body=3, seed=5, sigma3=0.3),
+    "li6_nmax2_3b": SystemConfig(3, 3, 2, max_body=3, seed=11, sigma3=0.1),
+}
+# FP64 bar: the oracle's own summation order differs from the kernel's, so
+# agreement is to rounding: 1e-12 of max|y|.
+FP64_TOL = 1e-12
+
+
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "no GPU visible"
+    return torch
+
+
+def gpu_matvec(op, x, accumulate=None):
+    torch = torch_dev()
+    xd = torch.as_tensor(x, dtype=torch.float64, device="cuda")
+    yd = (torch.as_tensor(accumulate, dtype=torch.float64, device="cuda").clone()
+          if accumulate is not None else torch.empty_like(xd))
+    op.matvec(xd, yd, accumulate=accumulate is not None)
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_fp64_matvec_matches_compiled_reference(name, golden_dir):
+    print(f"case {name}", flush=True)
+    g = np.load(os.path.join(golden_dir, f"{name}.npz"))
+    inp = build(CONFIGS[name])
+    with bacchus.Operator.from_inputs(inp) as op:
+        y = gpu_matvec(op, g["x"])
+    assert rel_err(y, g["y_ref"]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name,seed", [("he4_nmax4", 1), ("he4_nmax6", 2), ("he4_nmax8", 3)])
+def test_fp64_matvec_matches_oracle(name, seed):
+    print(f"case {name}", flush=True)
+    inp = build(SYSTEMS[name])
+    p = O.Problem.from_inputs(inp)
+    x = np.random.default_rng(seed).standard_normal(inp.dimension)
+    with bacchus.Operator.from_inputs(inp) as op:
+        y = gpu_matvec(op, x)
+        # accumulate: y0 + H x
+        y0 = np.random.default_rng(seed + 100).standard_normal(inp.dimension)
+        y_acc = gpu_matvec(op, x, accumulate=y0)
+    y_o = O.matvec(p, x)
+    assert rel_err(y, y_o) <= FP64_TOL
+    assert rel_err(y_acc, y_o + y0) <= FP64_TOL
+
+
+def test_force_scales_match_oracle():
+    print("case scales li6_nmax2_3b", flush=True)
+    inp = build(CONFIGS["li6_nmax2_3b"])
+    p = O.Problem.from_inputs(inp)
+    x = np.random.default_rng(9).standard_normal(inp.dimension)
+    with bacchus.Operator.from_inputs(inp) as op:
+        op.set_scales(1.0, 0.85, 1.15)
+        y = gpu_matvec(op, x)
+    y_o = O.matvec(p, x, scales=[1.0, 0.85, 1.15])
+    assert rel_err(y, y_o) <= FP64_TOL
+    y_plain = O.matvec(p, x)
+    assert rel_err(y_plain, y_o) > 1e-6  # the scales did act
+
+
+@pytest.mark.parametrize("precision,tol", [(bacchus.FP32, 2e-6), (bacchus.BF16, 2e-2)])
+def test_reduced_precision_storage_error_bounds(precision, tol):
+    """FP32/BF16-stored elements with FP32 accumulation: relative error of y
+    against the FP64 oracle stays within the storage precision's bound."""
+    print(f"case precision {precision}", flush=True)
+    inp = build(SYSTEMS["he4_nmax6"])
+    x = np.random.default_rng(4).standard_normal(inp.dimension)
+    y_o = O.matvec(O.Problem.from_inputs(inp), x)
+    with bacchus.Operator.from_inputs(inp, precision=precision) as op:
+        y = gpu_matvec(op, x)
+    err = rel_err(y, y_o)
+    assert err <= tol
+    assert err > 0.0  # it really is a reduced-precision path
+
+
+def test_file_drop_in_matches_compiled_reference(golden_dir):
+    """comb.txt + evaluation order + index_list_<id> + <id>_matrix_elements
+    + vec_<id> directories in, vec_<id> out (accumulating into the existing
+    output files like the reference)."""
+    print("case file drop-in he4_nmax4", flush=True)
+    g = np.load(os.path.join(golden_dir, "he4_nmax4.npz"))
+    inp = build(CONFIGS["he4_nmax4"])
+    d = tempfile.mkdtemp(prefix="bacchus_dropin_")
+    try:
+        inp.write(d)
+        off = inp.block_offsets
+        y0 = np.random.default_rng(1).standard_normal(inp.dimension)
+        for name, v in (("x", g["x"]), ("y", y0)):
+            os.makedirs(os.path.join(d, name))
+            for b in range(len(inp.blocks)):
+                v[off[b]:off[b + 1]].tofile(os.path.join(d, name, f"vec_{b + 1}"))
+        with bacchus.Operator.load(os.path.join(d, "comb.txt"), 2, 2,
+                                   os.path.join(d, "evaluation_order"),
+                                   os.path.join(d, "index_lists"),
+                                   os.path.join(d, "matrix_elements")) as op:
+            bacchus.run_matrix_vector_multiplication(os.path.join(d, "y"), os.path.join(d, "x"),
+                                                     op)
+        y = np.concatenate([np.fromfile(os.path.join(d, "y", f"vec_{b + 1}"))
+                            for b in range(len(inp.blocks))])
+        assert rel_err(y - y0, g["y_ref"]) <= FP64_TOL
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def test_c2_size_properties():
+    """At the 2+2 Nmax 8 size: linearity and symmetry (x.Hy = y.Hx)."""
+    print("case c2 properties", flush=True)
+    inp = build(SYSTEMS["he4_nmax8"])
+    rng = np.random.default_rng(8)
+    x, z = rng.standard_normal(inp.dimension), rng.standard_normal(inp.dimension)
+    with bacchus.Operator.from_inputs(inp) as op:
+        hx, hz = gpu_matvec(op, x), gpu_matvec(op, z)
+        hxz = gpu_matvec(op, 2.0 * x - 0.5 * z)
+    assert rel_err(hxz, 2.0 * hx - 0.5 * hz) <= 1e-13
+    assert abs(x @ hz - z @ hx) <= 1e-12 * np.abs(hx).max() * np.abs(z).sum()
+
+
+def test_empty_instruction_list_gives_zero():
+    print("case empty instructions", flush=True)
+    inp = build(CONFIGS["he4_nmax2_3b"])
+    torch = torch_dev()
+    with bacchus.Operator.from_arrays(inp.blocks[:, 4], inp.blocks[:, 5], [0], np.zeros(0),
+                                      [0], np.zeros(0), np.zeros(0), np.zeros(0),
+                                      np.zeros((0, 8))) as op:
+        x = torch.ones(inp.dimension, dtype=torch.float64, device="cuda")
+        y = torch.full_like(x, 7.0)
+        op.matvec(x, y)
+        torch.cuda.synchronize()
+        assert float(y.abs().max()) == 0.0
