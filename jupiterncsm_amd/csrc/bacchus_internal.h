@@ -54,7 +54,26 @@ struct VInstr {
 	int16_t force;     // 1, 2, 3 body
 };
 
+// One neutron-entry chunk of an NP instruction (one direction) against a
+// range of proton output rows.
+struct NPWork {
+	int64_t y_off;     // output block offset in y
+	int64_t x_off;     // input block offset in x
+	int64_t v_off;     // first matrix element of the block
+	int64_t dn;        // neutron matrix dimension (row stride of V)
+	uint32_t n_begin;  // first neutron entry of the chunk (absolute)
+	uint32_t p_rows;   // row pointer of proton output row 0
+	int32_t n_count;   // neutron entries in the chunk (<= kChunk)
+	int32_t nn_in;     // neutron states of the input block
+	int32_t nn_out;    // neutron states of the output block
+	int32_t po_begin;  // proton output rows [po_begin, po_end)
+	int32_t po_end;
+	int32_t force;     // 1, 2, 3 body
+};
+
 constexpr int kChunk = 256;
+constexpr uint32_t kProtonRange = 4096;
+constexpr int kXcds = 8;
 
 }  // namespace bacchus
 
@@ -70,6 +89,9 @@ struct bacchus_hip_operator {
 	uint2 *d_entries = nullptr;     // {other index, matrix index|sign}
 	bacchus::VInstr *d_vinstr = nullptr;
 	bacchus::WorkItem *d_work = nullptr;
+	uint32_t *d_keys = nullptr;     // row of every CSR entry
+	bacchus::NPWork *d_npwork = nullptr;
+	int64_t num_npwork = 0;
 	float *d_xf = nullptr;          // FP32 copy of x (FP32/BF16 modes)
 	int64_t num_work = 0;
 	int64_t num_vinstr = 0;

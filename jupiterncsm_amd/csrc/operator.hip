@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -147,6 +148,77 @@ __global__ __launch_bounds__(kChunk) void matvec_kernel(
 	atomicAdd(&y[w.y_off + t], (double)acc);
 }
 
+// XCD-aware order: the hardware deals workgroups round-robin over the
+// XCDs, so logical item (xcd * per + k) is given to the k-th block of that
+// XCD and neighbouring items (same element block) share one L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n)
+{
+	const int64_t per = n / kXcds;
+	return (b % kXcds) * per + b / kXcds;
+}
+
+// NP product (c.f. matrix_vector_multiplication.c:134-210 and the
+// off-diagonal form :364-462).  Lane i holds neutron entry i of a chunk of
+// the list copy sorted by output row (then element index); the proton
+// entries of one output row p_o are wave-uniform (scalar loads):
+//   acc_i = s_i * sum_{j in Lp(p_o)} s_j V[m_i + Dn m_j] x[n_i + Nn_in p_j]
+// then lanes with equal output row n_o are summed by a segmented
+// reduction and the first lane of each run adds the sum to y[n_o + Nn_out p_o].
+template <typename VT>
+__global__ __launch_bounds__(kChunk) void np_kernel(
+	const NPWork *__restrict__ work, int64_t nwork,
+	const uint32_t *__restrict__ rowptr, const uint2 *__restrict__ ent,
+	const uint32_t *__restrict__ keys, const VT *__restrict__ V,
+	const typename Elem<VT>::x_t *__restrict__ x, double *__restrict__ y,
+	double s1, double s2, double s3)
+{
+	using acc_t = typename Elem<VT>::acc_t;
+	const NPWork w = work[xcd_remap(blockIdx.x, nwork)];
+	const int t = threadIdx.x;
+	const int lane = t & 63;
+	const bool valid = t < w.n_count;
+	const uint32_t k = w.n_begin + (valid ? t : 0);
+	const uint2 ne = ent[k];
+	const uint32_t key = valid ? keys[k] : 0xFFFFFFFFu;
+	// step s of the segmented reduction adds lane + 2^s when it is in the
+	// same run of equal keys (runs are contiguous: the copy is sorted)
+	uint32_t add = 0;
+#pragma unroll
+	for (int s = 0; s < 6; s++) {
+		const uint32_t other = __shfl_down(key, 1 << s, 64);
+		if (lane + (1 << s) < 64 && other == key)
+			add |= 1u << s;
+	}
+	const uint32_t prev = __shfl_up(key, 1, 64);
+	const bool head = valid && (lane == 0 || prev != key);
+	const acc_t sc = apply_sign(acc_t(w.force == 1 ? s1 : (w.force == 2 ? s2 : s3)), ne.y);
+	const VT *vrow0 = V + w.v_off + (ne.y & kIndexMask);
+	const auto *xcol0 = x + w.x_off + ne.x;
+	double *ycol0 = y + w.y_off + key;
+	for (int po = w.po_begin; po < w.po_end; po++) {
+		const uint32_t jb = rowptr[w.p_rows + po];
+		const uint32_t je = rowptr[w.p_rows + po + 1];
+		if (jb == je)
+			continue;
+		acc_t acc = 0;
+		for (uint32_t j = jb; j < je; j++) {
+			const uint2 pe = ent[j];
+			const acc_t v = Elem<VT>::load(vrow0, w.dn * (int64_t)(pe.y & kIndexMask));
+			const acc_t xv = acc_t(xcol0[(int64_t)w.nn_in * pe.x]);
+			acc += apply_sign(v, pe.y) * xv;
+		}
+		acc *= sc;
+#pragma unroll
+		for (int s = 0; s < 6; s++) {
+			const acc_t o = __shfl_down(acc, 1 << s, 64);
+			if (add & (1u << s))
+				acc += o;
+		}
+		if (head)
+			atomicAdd(ycol0 + (int64_t)w.nn_out * po, (double)acc);
+	}
+}
+
 __global__ void scale_kernel(double *y, int64_t n, double s)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,6 +231,20 @@ __global__ void to_float_kernel(const double *x, float *xf, int64_t n)
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
 		xf[i] = (float)x[i];
+}
+
+template <typename VT>
+void launch(const bacchus_hip_operator *op, const VT *V, const typename Elem<VT>::x_t *x,
+	    double *y, double s1, double s2, double s3, hipStream_t s)
+{
+	if (op->num_npwork)
+		hipLaunchKernelGGL(np_kernel<VT>, dim3(op->num_npwork), dim3(kChunk), 0, s,
+				   op->d_npwork, op->num_npwork, op->d_rowptr, op->d_entries,
+				   op->d_keys, V, x, y, s1, s2, s3);
+	if (op->num_work)
+		hipLaunchKernelGGL(matvec_kernel<VT>, dim3(op->num_work), dim3(kChunk), 0, s,
+				   op->d_work, op->d_vinstr, op->d_rowptr, op->d_entries, V, x, y,
+				   s1, s2, s3);
 }
 
 uint16_t to_bf16(double v)
@@ -215,6 +301,8 @@ void bacchus_hip_operator_free(bacchus_hip_operator_t *op)
 	(void)hipFree(op->d_entries);
 	(void)hipFree(op->d_vinstr);
 	(void)hipFree(op->d_work);
+	(void)hipFree(op->d_keys);
+	(void)hipFree(op->d_npwork);
 	(void)hipFree(op->d_xf);
 	delete op;
 }
@@ -382,9 +470,12 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 						 (long long)use[l].n_out));
 	}
 
-	// --- CSR copies of every used list: by output and by input index ---
+	// --- CSR copies of every used list: by output and by input index,
+	// each row ordered by matrix index (neighbouring lanes then read
+	// neighbouring elements); keys[k] = row of entry k ---
 	std::vector<uint32_t> rowptr;
 	std::vector<uint2> entries;
+	std::vector<uint32_t> keys;
 	std::vector<int64_t> rows_by_out(nl, -1), rows_by_in(nl, -1);
 	int64_t used_entries = 0;
 	for (int64_t l = 0; l < nl; l++) {
@@ -405,6 +496,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			for (int64_t r = 0; r <= nrows; r++)
 				rowptr.push_back(base + count[r]);
 			entries.resize(entries.size() + (b - a));
+			keys.resize(keys.size() + (b - a));
 			std::vector<uint32_t> fill(count.begin(), count.end() - 1);
 			for (int64_t k = a; k < b; k++) {
 				const int32_t *e3 = &p->list_entries[3 * k];
@@ -412,17 +504,61 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				uint2 en;
 				en.x = (uint32_t)e3[1 - key];
 				en.y = (uint32_t)e3[2];
+				keys[base + fill[r]] = r;
 				entries[base + fill[r]++] = en;
 			}
+			for (int64_t r = 0; r < nrows; r++)
+				std::sort(entries.begin() + base + count[r],
+					  entries.begin() + base + count[r + 1],
+					  [](const uint2 &u, const uint2 &w) {
+						  return (u.y & 0x7FFFFFFFu) < (w.y & 0x7FFFFFFFu);
+					  });
 			if (entries.size() > 0xFFFFFFFFull || rowptr.size() > 0xFFFFFFFFull)
 				return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
 		}
 	}
 	op->list_entry_count = used_entries;
+	// Optional ordering of the neutron lists of NP instructions by element
+	// index alone (coalesced V rows, shorter reduction runs); rows stay in
+	// keys[] so the NP kernel's segmented reduction is unchanged.
+	const char *order_env = getenv("BACCHUS_HIP_NP_ORDER");
+	const bool element_order = order_env && strcmp(order_env, "element") == 0;
+	std::vector<int64_t> elem_by_out(nl, -1), elem_by_in(nl, -1);
+	if (element_order) {
+		for (int64_t i = 0; i < ni; i++) {
+			const bacchus_hip_instruction_t &I = p->instructions[i];
+			if (I.type != 3 || elem_by_out[I.neutron_list] >= 0)
+				continue;
+			const int64_t l = I.neutron_list;
+			for (int dir = 0; dir < 2; dir++) {
+				const int64_t r0 = (dir == 0 ? rows_by_out : rows_by_in)[l];
+				const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
+				const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
+				std::vector<uint32_t> perm(b - a);
+				std::iota(perm.begin(), perm.end(), a);
+				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+					return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
+				});
+				(dir == 0 ? elem_by_out : elem_by_in)[l] = (int64_t)entries.size();
+				for (uint32_t k : perm) {
+					entries.push_back(entries[k]);
+					keys.push_back(keys[k]);
+				}
+			}
+		}
+		if (entries.size() > 0xFFFFFFFFull)
+			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+	}
 
-	// --- virtual instructions grouped by output block ---
+	// --- virtual instructions: N and P grouped by output block; NP split
+	// into (neutron-entry chunk x proton output-row range) work items ---
 	std::vector<std::vector<VInstr>> per_block(nb);
 	std::vector<std::vector<double>> per_block_cost(nb);
+	struct NPV {
+		NPWork proto;
+		int64_t n_base, n_len, p_rows, n_pout;
+	};
+	std::vector<NPV> npv;
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		const int force = p->matrix_force ? p->matrix_force[I.matrix_block] : 2;
@@ -433,25 +569,47 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		for (int dir = 0; dir < (I.block_in == I.block_out ? 1 : 2); dir++) {
 			const int64_t bin = dir == 0 ? I.block_in : I.block_out;
 			const int64_t bout = dir == 0 ? I.block_out : I.block_in;
+			const auto &rows = dir == 0 ? rows_by_out : rows_by_in;
+			const int f = std::min(3, std::max(1, force));
+			if (I.type == 3) {
+				if (ln == 0 || lp == 0)
+					continue;
+				NPV q{};
+				q.proto.y_off = op->block_off[bout];
+				q.proto.x_off = op->block_off[bin];
+				q.proto.v_off = p->matrix_offsets[I.matrix_block];
+				q.proto.dn = vdn[I.matrix_block];
+				q.proto.nn_in = (int32_t)op->block_nn[bin];
+				q.proto.nn_out = (int32_t)op->block_nn[bout];
+				q.proto.force = f;
+				const int64_t nr = rows[I.neutron_list];
+				q.n_base = element_order
+					? (dir == 0 ? elem_by_out : elem_by_in)[I.neutron_list]
+					: (int64_t)rowptr[nr];
+				q.n_len = ln;
+				q.p_rows = rows[I.proton_list];
+				q.n_pout = op->block_np[bout];
+				npv.push_back(q);
+				continue;
+			}
 			VInstr v{};
 			v.x_off = op->block_off[bin];
 			v.v_off = p->matrix_offsets[I.matrix_block];
-			v.dn = I.type == 3 ? vdn[I.matrix_block] : 0;
+			v.dn = 0;
 			v.nn_in = (int32_t)op->block_nn[bin];
 			v.type = (int16_t)I.type;
-			v.force = (int16_t)std::min(3, std::max(1, force));
-			if (I.type != 2)
-				v.n_rows = (uint32_t)(dir == 0 ? rows_by_out : rows_by_in)[I.neutron_list];
-			if (I.type != 1)
-				v.p_rows = (uint32_t)(dir == 0 ? rows_by_out : rows_by_in)[I.proton_list];
+			v.force = (int16_t)f;
+			if (I.type == 1)
+				v.n_rows = (uint32_t)rows[I.neutron_list];
+			else
+				v.p_rows = (uint32_t)rows[I.proton_list];
 			const double m = I.type == 1 ? double(ln) * op->block_np[bin]
-					 : I.type == 2 ? double(lp) * op->block_nn[bin]
-						       : double(ln) * lp;
+						     : double(lp) * op->block_nn[bin];
 			per_block[bout].push_back(v);
 			per_block_cost[bout].push_back(m);
 		}
 	}
-	// --- work items: 256-element chunks x instruction ranges ---
+	// --- N/P work items: 256-element chunks x instruction ranges ---
 	std::vector<VInstr> vinstr;
 	std::vector<WorkItem> work;
 	int num_cu = 256;
@@ -462,10 +620,16 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		    hipGetDeviceProperties(&prop, dev) == hipSuccess)
 			num_cu = prop.multiProcessorCount;
 	}
-	const double target = std::max(2.0e4, mults / (num_cu * 32.0));
+	double np_mults = 0;
+	for (auto &c : per_block_cost)
+		for (double v : c)
+			np_mults += v;
+	const double target = std::max(2.0e4, np_mults / (num_cu * 32.0));
 	for (int64_t b = 0; b < nb; b++) {
 		auto &vs = per_block[b];
 		auto &cs = per_block_cost[b];
+		if (vs.empty())
+			continue;
 		std::vector<int64_t> order(vs.size());
 		std::iota(order.begin(), order.end(), 0);
 		std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) {
@@ -502,6 +666,46 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	}
 	op->num_work = (int64_t)work.size();
 	op->num_vinstr = (int64_t)vinstr.size();
+
+	// --- NP work items, grouped by element block (V locality), then
+	// proton output-row range, then neutron chunk ---
+	std::stable_sort(npv.begin(), npv.end(), [](const NPV &a, const NPV &b) {
+		return a.proto.v_off < b.proto.v_off;
+	});
+	std::vector<NPWork> npwork;
+	for (const NPV &q : npv) {
+		// proton output rows split so a work item sees <= kProtonRange
+		// proton entries (a single long row stays whole)
+		int32_t po0 = 0;
+		while (po0 < q.n_pout) {
+			int32_t po1 = po0;
+			uint32_t taken = 0;
+			while (po1 < q.n_pout) {
+				const uint32_t len = rowptr[q.p_rows + po1 + 1] - rowptr[q.p_rows + po1];
+				if (po1 > po0 && taken + len > kProtonRange)
+					break;
+				taken += len;
+				po1++;
+			}
+			if (taken > 0) {
+				for (int64_t c = 0; c < q.n_len; c += kChunk) {
+					NPWork w = q.proto;
+					w.n_begin = (uint32_t)(q.n_base + c);
+					w.n_count = (int32_t)std::min<int64_t>(kChunk, q.n_len - c);
+					w.p_rows = (uint32_t)q.p_rows;
+					w.po_begin = po0;
+					w.po_end = po1;
+					npwork.push_back(w);
+				}
+			}
+			po0 = po1;
+		}
+	}
+	// pad to a multiple of the XCD count so the XCD-aware block remap
+	// is a bijection (empty items do nothing)
+	while (npwork.size() % kXcds)
+		npwork.push_back(NPWork{});
+	op->num_npwork = (int64_t)npwork.size();
 
 	// --- statistics: compulsory bytes of referenced arrays ---
 	std::vector<char> vref(nv, 0);
@@ -545,6 +749,8 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		return fail(rc);
 	if ((rc = upload((void **)&op->d_rowptr, rowptr.data(), rowptr.size() * 4)) ||
 	    (rc = upload((void **)&op->d_entries, entries.data(), entries.size() * 8)) ||
+	    (rc = upload((void **)&op->d_keys, keys.data(), keys.size() * 4)) ||
+	    (rc = upload((void **)&op->d_npwork, npwork.data(), npwork.size() * sizeof(NPWork))) ||
 	    (rc = upload((void **)&op->d_vinstr, vinstr.data(), vinstr.size() * sizeof(VInstr))) ||
 	    (rc = upload((void **)&op->d_work, work.data(), work.size() * sizeof(WorkItem))))
 		return fail(rc);
@@ -566,23 +772,15 @@ int bacchus_hip_operator_matvec(bacchus_hip_operator_t *op, const double *x,
 	const int64_t blocks = (n + threads - 1) / threads;
 	if (!accumulate)
 		hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(threads), 0, s, y, n, 0.0);
-	if (op->num_work == 0)
-		return 0;
 	const double s1 = op->scales[0], s2 = op->scales[1], s3 = op->scales[2];
 	if (op->precision == BACCHUS_HIP_FP64) {
-		hipLaunchKernelGGL(matvec_kernel<double>, dim3(op->num_work), dim3(kChunk), 0, s,
-				   op->d_work, op->d_vinstr, op->d_rowptr, op->d_entries,
-				   (const double *)op->d_v, x, y, s1, s2, s3);
+		launch<double>(op, (const double *)op->d_v, x, y, s1, s2, s3, s);
 	} else {
 		hipLaunchKernelGGL(to_float_kernel, dim3(blocks), dim3(threads), 0, s, x, op->d_xf, n);
 		if (op->precision == BACCHUS_HIP_FP32)
-			hipLaunchKernelGGL(matvec_kernel<float>, dim3(op->num_work), dim3(kChunk), 0, s,
-					   op->d_work, op->d_vinstr, op->d_rowptr, op->d_entries,
-					   (const float *)op->d_v, op->d_xf, y, s1, s2, s3);
+			launch<float>(op, (const float *)op->d_v, op->d_xf, y, s1, s2, s3, s);
 		else
-			hipLaunchKernelGGL(matvec_kernel<uint16_t>, dim3(op->num_work), dim3(kChunk), 0,
-					   s, op->d_work, op->d_vinstr, op->d_rowptr, op->d_entries,
-					   (const uint16_t *)op->d_v, op->d_xf, y, s1, s2, s3);
+			launch<uint16_t>(op, (const uint16_t *)op->d_v, op->d_xf, y, s1, s2, s3, s);
 	}
 	BACCHUS_HIP_CHECK(hipGetLastError());
 	return 0;
