@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only run this many FP64 matvecs (for profilers) and exit")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only copy this many MiB device-to-device 3 times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     return ap.parse_args()
 
 
@@ -76,6 +79,15 @@ def main():
     from jupiterncsm_amd.inputs import SYSTEMS, build
 
     assert torch.cuda.is_available(), "no GPU"
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB x 3")
+        return
     t0 = time.time()
     inputs = build(SYSTEMS[args.workload])
     log(f"inputs {args.workload}: {inputs.summary()} in {time.time() - t0:.1f}s")
@@ -150,7 +162,7 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         cores = len(os.sched_getaffinity(0))
-        prob = O.Problem.from_inputs(build(SYSTEMS[args.workload]))
+        prob = O.Problem.from_inputs(inputs)  # views into `inputs`, kept alive
         xv = x.cpu().numpy()
         O.matvec(prob, xv, threads=cores)
         t = time.perf_counter()

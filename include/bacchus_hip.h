@@ -158,6 +158,29 @@ int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 			double *ritz_vector,
 			bacchus_hip_lanczos_result_t *result);
 
+/* Learned Lanczos start (no counterpart in the reference): an MLP
+ * in -> hidden (x depth, GELU) -> out with BF16 weights on MFMA, FP32
+ * accumulation and FP32 master weights trained with Adam on the device.
+ * Parameters are all weight matrices (out x in, row-major, layer order)
+ * followed by all biases.  x, t, y are device FP32 arrays, row-major
+ * (M rows). */
+typedef struct bacchus_hip_mlp bacchus_hip_mlp_t;
+int bacchus_hip_mlp_create(int in, int hidden, int depth, int out,
+			   uint64_t seed, bacchus_hip_mlp_t **mlp);
+void bacchus_hip_mlp_free(bacchus_hip_mlp_t *mlp);
+int64_t bacchus_hip_mlp_num_params(const bacchus_hip_mlp_t *mlp);
+int bacchus_hip_mlp_get_params(const bacchus_hip_mlp_t *mlp, float *host);
+int bacchus_hip_mlp_set_params(bacchus_hip_mlp_t *mlp, const float *host);
+int bacchus_hip_mlp_forward(bacchus_hip_mlp_t *mlp, const float *x, int M,
+			    float *y, void *stream);
+/* gradients of the mean squared error against t (host copy) */
+int bacchus_hip_mlp_gradients(bacchus_hip_mlp_t *mlp, const float *x,
+			      const float *t, int M, float *grads_host,
+			      float *loss_host, void *stream);
+int bacchus_hip_mlp_train_step(bacchus_hip_mlp_t *mlp, const float *x,
+			       const float *t, int M, float lr,
+			       float *loss_host, void *stream);
+
 const char *bacchus_hip_last_error(void);
 
 #ifdef __cplusplus

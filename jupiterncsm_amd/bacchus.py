@@ -42,7 +42,9 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_operator_dimension", "bacchus_hip_operator_set_scales",
     "bacchus_hip_operator_matvec", "bacchus_hip_operator_stats",
     "bacchus_hip_run_matrix_vector_multiplication", "bacchus_hip_lanczos",
-    "bacchus_hip_last_error",
+    "bacchus_hip_last_error", "bacchus_hip_mlp_create", "bacchus_hip_mlp_free",
+    "bacchus_hip_mlp_num_params", "bacchus_hip_mlp_get_params", "bacchus_hip_mlp_set_params",
+    "bacchus_hip_mlp_forward", "bacchus_hip_mlp_gradients", "bacchus_hip_mlp_train_step",
 )
 
 
@@ -296,3 +298,92 @@ def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
     k = int(res.iterations)
     return LanczosResult(ev[:min(k, num_eigenvalues)], alpha[:k], beta[:k], k,
                          res.matvec_seconds, res.total_seconds)
+
+
+class MLP:
+    """Learned-start MLP on the device (BF16 MFMA forward/backward, FP32
+    master weights, Adam).  Inputs and targets are float32 torch tensors on
+    the GPU, row-major (M rows)."""
+
+    def __init__(self, inputs=2, hidden=256, depth=3, outputs=64, seed=1):
+        lib = library()
+        _bind_mlp(lib)
+        h = ctypes.c_void_p()
+        _check(lib.bacchus_hip_mlp_create(inputs, hidden, depth, outputs, seed, ctypes.byref(h)))
+        self._h = h
+        self.shape = (inputs, hidden, depth, outputs)
+        self.num_params = int(lib.bacchus_hip_mlp_num_params(h))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            library().bacchus_hip_mlp_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, dtype=np.float32)
+        _check(library().bacchus_hip_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(library().bacchus_hip_mlp_set_params(self._h, _ptr(p)))
+
+    def forward(self, x, stream=None):
+        import torch
+        x = x.contiguous()
+        y = torch.empty(x.shape[0], self.shape[3], dtype=torch.float32, device=x.device)
+        _check(library().bacchus_hip_mlp_forward(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                 x.shape[0], ctypes.c_void_p(y.data_ptr()),
+                                                 _stream_handle(stream)))
+        return y
+
+    def gradients(self, x, t, stream=None):
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(library().bacchus_hip_mlp_gradients(
+            self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()), x.shape[0],
+            _ptr(g), ctypes.byref(loss), _stream_handle(stream)))
+        return float(loss.value), g
+
+    def train_step(self, x, t, lr=1e-3, stream=None, want_loss=False):
+        loss = ctypes.c_float()
+        _check(library().bacchus_hip_mlp_train_step(
+            self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(t.data_ptr()), x.shape[0],
+            float(lr), ctypes.byref(loss) if want_loss else None, _stream_handle(stream)))
+        return float(loss.value) if want_loss else None
+
+
+_MLP_BOUND = False
+
+
+def _bind_mlp(lib):
+    global _MLP_BOUND
+    if _MLP_BOUND:
+        return
+    vp, c_int = ctypes.c_void_p, ctypes.c_int
+    lib.bacchus_hip_mlp_create.restype = c_int
+    lib.bacchus_hip_mlp_create.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_uint64,
+                                           ctypes.POINTER(vp)]
+    lib.bacchus_hip_mlp_free.restype = None
+    lib.bacchus_hip_mlp_free.argtypes = [vp]
+    lib.bacchus_hip_mlp_num_params.restype = ctypes.c_int64
+    lib.bacchus_hip_mlp_num_params.argtypes = [vp]
+    lib.bacchus_hip_mlp_get_params.restype = c_int
+    lib.bacchus_hip_mlp_get_params.argtypes = [vp, vp]
+    lib.bacchus_hip_mlp_set_params.restype = c_int
+    lib.bacchus_hip_mlp_set_params.argtypes = [vp, vp]
+    lib.bacchus_hip_mlp_forward.restype = c_int
+    lib.bacchus_hip_mlp_forward.argtypes = [vp, vp, c_int, vp, vp]
+    lib.bacchus_hip_mlp_gradients.restype = c_int
+    lib.bacchus_hip_mlp_gradients.argtypes = [vp, vp, vp, c_int, vp, ctypes.POINTER(ctypes.c_float), vp]
+    lib.bacchus_hip_mlp_train_step.restype = c_int
+    lib.bacchus_hip_mlp_train_step.argtypes = [vp, vp, vp, c_int, ctypes.c_float,
+                                               ctypes.POINTER(ctypes.c_float), vp]
+    _MLP_BOUND = True

@@ -27,52 +27,57 @@ int set_error(const char *func, const char *file, int line, const char *fmt,
 					    hipGetErrorString(err_));        \
 	} while (0)
 
-// One output-element chunk of one basis block and the range of virtual
-// instructions it sums over.
-struct WorkItem {
-	int64_t y_off;     // offset of the chunk's first element in y
-	int32_t count;     // elements in the chunk (<= kChunk)
-	int32_t nn_out;    // neutron states of the output block
-	int32_t elem0;     // block-local index of the chunk's first element
-	int32_t vi_begin;  // virtual instruction range
-	int32_t vi_end;
-	int32_t pad;
-};
-
-// An instruction applied in one direction: forward (H_out,in x_in) or, for
-// off-diagonal instructions, transposed (H_out,in^T x_out into y_in).  Row
-// pointers select the CSR copy of each list sorted by this direction's
-// output index.
-struct VInstr {
+// One direction of one instruction, as the tile kernel applies it:
+// forward (H_out,in x_in) or, for off-diagonal instructions, transposed
+// (H_out,in^T x_out into y_in) using the list copies sorted by input index.
+struct TInstr {
 	int64_t x_off;     // offset of the input block in x
 	int64_t v_off;     // first matrix element
-	int64_t dn;        // neutron matrix dimension (NP stride)
-	uint32_t n_rows;   // first row pointer of the neutron list copy
-	uint32_t p_rows;   // first row pointer of the proton list copy
+	int64_t dn;        // neutron matrix dimension (NP row stride of V)
+	uint32_t n_base;   // NP: first entry of the element-ordered neutron
+			   // copy; N: row pointer of neutron row 0
+	uint32_t n_len;    // NP: neutron entries
+	uint32_t p_rows;   // NP, P: row pointer of proton row 0
 	int32_t nn_in;     // neutron states of the input block
 	int16_t type;      // 1 N, 2 P, 3 NP
 	int16_t force;     // 1, 2, 3 body
+	int32_t pad;
 };
 
-// One neutron-entry chunk of an NP instruction (one direction) against a
-// range of proton output rows.
-struct NPWork {
-	int64_t y_off;     // output block offset in y
-	int64_t x_off;     // input block offset in x
-	int64_t v_off;     // first matrix element of the block
-	int64_t dn;        // neutron matrix dimension (row stride of V)
-	uint32_t n_begin;  // first neutron entry of the chunk (absolute)
-	uint32_t p_rows;   // row pointer of proton output row 0
-	int32_t n_count;   // neutron entries in the chunk (<= kChunk)
-	int32_t nn_in;     // neutron states of the input block
-	int32_t nn_out;    // neutron states of the output block
-	int32_t po_begin;  // proton output rows [po_begin, po_end)
+// One output tile: rows [po_begin, po_end) of a basis block (all neutron
+// states) and the range of virtual instructions summed into it.
+struct TileWork {
+	int64_t y_off;     // offset of the block in y
+	int32_t nn_out;    // neutron states of the block
+	int32_t po_begin;
 	int32_t po_end;
-	int32_t force;     // 1, 2, 3 body
+	int32_t vi_begin;
+	int32_t vi_end;
+	int32_t shared;    // other tiles cover the same rows: flush atomically
+	int32_t t_begin;   // tasks [t_begin, t_end)
+	int32_t t_end;
+	int32_t pad;
 };
 
-constexpr int kChunk = 256;
-constexpr uint32_t kProtonRange = 4096;
+// One unit of a tile's work: entries [jb, je) of an N instruction over all
+// rows (r = -1), or proton entries [jb, je) (rows of the tile, in row
+// order) of a P/NP instruction with the range [e0, e1) of neutron entries
+// (NP) or neutron states (P) it covers.
+struct Task {
+	int32_t vi;
+	int32_t r;
+	uint32_t jb;
+	uint32_t je;
+	uint32_t e0;
+	uint32_t e1;
+	uint32_t pad0;
+	uint32_t pad1;
+};
+
+constexpr int kTileThreads = 256;
+constexpr int kTileMaxElems = 2048;  // doubles of LDS per tile (16 KiB)
+constexpr uint32_t kNTaskEntries = 2048;  // list entries per N task
+constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
 constexpr int kXcds = 8;
 
 }  // namespace bacchus
@@ -87,17 +92,19 @@ struct bacchus_hip_operator {
 	void *d_v = nullptr;            // elements in storage precision
 	uint32_t *d_rowptr = nullptr;   // CSR row pointers (absolute)
 	uint2 *d_entries = nullptr;     // {other index, matrix index|sign}
-	bacchus::VInstr *d_vinstr = nullptr;
-	bacchus::WorkItem *d_work = nullptr;
-	uint32_t *d_keys = nullptr;     // row of every CSR entry
-	bacchus::NPWork *d_npwork = nullptr;
-	int64_t num_npwork = 0;
+	uint32_t *d_keys = nullptr;     // row of every entry
+	bacchus::TInstr *d_tinstr = nullptr;
+	bacchus::TileWork *d_tiles = nullptr;
+	bacchus::Task *d_tasks = nullptr;
+	int64_t num_tasks = 0;
 	float *d_xf = nullptr;          // FP32 copy of x (FP32/BF16 modes)
-	int64_t num_work = 0;
-	int64_t num_vinstr = 0;
+	int64_t num_tiles = 0;
+	int64_t num_tinstr = 0;
+	size_t tile_lds_bytes = 0;
 	// statistics
 	double algorithmic_bytes = 0;
 	double flops = 0;
 	int64_t element_count = 0;
+	int64_t compact_elements = 0;  // elements of the dense sub-blocks
 	int64_t list_entry_count = 0;
 };

@@ -20,7 +20,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <numeric>
+#include <tuple>
 
 namespace bacchus {
 
@@ -76,146 +78,203 @@ template <> struct Elem<uint16_t> {
 	}
 };
 
-// One lane per output element.  For element (n_o, p_o) of the output block:
-//   N  : sum_{i in Ln(n_o)}  s_i V[m_i] x[n_i + Nn_in p_o]       (c.f. :44-76)
-//   P  : sum_{j in Lp(p_o)}  s_j V[m_j] x[n_o + Nn_in p_j]       (c.f. :99-131)
-//   NP : sum_{j in Lp(p_o)} s_j sum_{i in Ln(n_o)} s_i V[m_i + Dn m_j]
-//                                      x[n_i + Nn_in p_j]        (c.f. :164-209)
-// The N and P forms skip |V| < 1e-12 as the reference does (:52, :107).
+// Owner-computes tile kernel.  A workgroup owns an output tile: the rows
+// p_o in [po_begin, po_end) of one basis block, all neutron states, held in
+// LDS.  It applies every virtual instruction that writes the block (the
+// instruction itself, or for off-diagonal instructions its transpose,
+// scheduler.c:310-418) and writes the tile to y once.
+//   NP (c.f. matrix_vector_multiplication.c:134-210): for each row p_o,
+//     threads over neutron entries i (ordered by element index):
+//     tile[p_o][n_o(i)] += s_i sum_{j in Lp(p_o)} s_j W[c_i + |C_n| c_j] x[n_i + Nn_in p_j]
+//     with W the instruction's dense element sub-block and c the compact
+//     element indices (see bacchus_hip_operator_create)
+//   N  (:20-77): thread per tile element, CSR row of n_o;
+//     tile[p_o][n_o] += sum_{i in Ln(n_o)} s_i V[m_i] x[n_i + Nn_in p_o]
+//   P  (:79-132): thread per tile element, CSR row of p_o;
+//     tile[p_o][n_o] += sum_{j in Lp(p_o)} s_j V[m_j] x[n_o + Nn_in p_j]
+// N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
-__global__ __launch_bounds__(kChunk) void matvec_kernel(
-	const WorkItem *__restrict__ work, const VInstr *__restrict__ vins,
-	const uint32_t *__restrict__ rowptr, const uint2 *__restrict__ ent,
-	const VT *__restrict__ V, const typename Elem<VT>::x_t *__restrict__ x,
-	double *__restrict__ y, double s1, double s2, double s3)
-{
-	using acc_t = typename Elem<VT>::acc_t;
-	const WorkItem w = work[blockIdx.x];
-	const int t = threadIdx.x;
-	if (t >= w.count)
-		return;
-	const int e = w.elem0 + t;
-	const int no = e % w.nn_out;
-	const int po = e / w.nn_out;
-	acc_t acc = 0;
-	for (int vi = w.vi_begin; vi < w.vi_end; vi++) {
-		const VInstr I = vins[vi];
-		const acc_t sc = acc_t(I.force == 1 ? s1 : (I.force == 2 ? s2 : s3));
-		acc_t part = 0;
-		if (I.type == 1) {
-			const uint32_t b = rowptr[I.n_rows + no];
-			const uint32_t f = rowptr[I.n_rows + no + 1];
-			const auto *xc = x + I.x_off + (int64_t)I.nn_in * po;
-			for (uint32_t k = b; k < f; k++) {
-				const uint2 en = ent[k];
-				const acc_t v = Elem<VT>::load(V, I.v_off + (en.y & kIndexMask));
-				if (fabs(v) < acc_t(1e-12))
-					continue;
-				part += apply_sign(v, en.y) * acc_t(xc[en.x]);
-			}
-		} else if (I.type == 2) {
-			const uint32_t b = rowptr[I.p_rows + po];
-			const uint32_t f = rowptr[I.p_rows + po + 1];
-			const auto *xc = x + I.x_off + no;
-			for (uint32_t k = b; k < f; k++) {
-				const uint2 en = ent[k];
-				const acc_t v = Elem<VT>::load(V, I.v_off + (en.y & kIndexMask));
-				if (fabs(v) < acc_t(1e-12))
-					continue;
-				part += apply_sign(v, en.y) *
-					acc_t(xc[(int64_t)I.nn_in * en.x]);
-			}
-		} else {
-			const uint32_t pb = rowptr[I.p_rows + po];
-			const uint32_t pf = rowptr[I.p_rows + po + 1];
-			const uint32_t nb = rowptr[I.n_rows + no];
-			const uint32_t nf = rowptr[I.n_rows + no + 1];
-			for (uint32_t j = pb; j < pf; j++) {
-				const uint2 pe = ent[j];
-				const int64_t vrow = I.v_off + I.dn * (pe.y & kIndexMask);
-				const auto *xc = x + I.x_off + (int64_t)I.nn_in * pe.x;
-				acc_t inner = 0;
-				for (uint32_t k = nb; k < nf; k++) {
-					const uint2 ne = ent[k];
-					const acc_t v = Elem<VT>::load(V, vrow + (ne.y & kIndexMask));
-					inner += apply_sign(v, ne.y) * acc_t(xc[ne.x]);
-				}
-				part += apply_sign(inner, pe.y);
-			}
-		}
-		acc += sc * part;
-	}
-	atomicAdd(&y[w.y_off + t], (double)acc);
-}
-
-// XCD-aware order: the hardware deals workgroups round-robin over the
-// XCDs, so logical item (xcd * per + k) is given to the k-th block of that
-// XCD and neighbouring items (same element block) share one L2.
-__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n)
-{
-	const int64_t per = n / kXcds;
-	return (b % kXcds) * per + b / kXcds;
-}
-
-// NP product (c.f. matrix_vector_multiplication.c:134-210 and the
-// off-diagonal form :364-462).  Lane i holds neutron entry i of a chunk of
-// the list copy sorted by output row (then element index); the proton
-// entries of one output row p_o are wave-uniform (scalar loads):
-//   acc_i = s_i * sum_{j in Lp(p_o)} s_j V[m_i + Dn m_j] x[n_i + Nn_in p_j]
-// then lanes with equal output row n_o are summed by a segmented
-// reduction and the first lane of each run adds the sum to y[n_o + Nn_out p_o].
-template <typename VT>
-__global__ __launch_bounds__(kChunk) void np_kernel(
-	const NPWork *__restrict__ work, int64_t nwork,
+__global__ __launch_bounds__(kTileThreads) void tile_kernel(
+	const TileWork *__restrict__ work, int64_t nwork, const Task *__restrict__ tasks,
+	const TInstr *__restrict__ ins,
 	const uint32_t *__restrict__ rowptr, const uint2 *__restrict__ ent,
 	const uint32_t *__restrict__ keys, const VT *__restrict__ V,
-	const typename Elem<VT>::x_t *__restrict__ x, double *__restrict__ y,
-	double s1, double s2, double s3)
+	const typename Elem<VT>::x_t *__restrict__ x, double *__restrict__ y, double s1,
+	double s2, double s3)
 {
 	using acc_t = typename Elem<VT>::acc_t;
-	const NPWork w = work[xcd_remap(blockIdx.x, nwork)];
-	const int t = threadIdx.x;
-	const int lane = t & 63;
-	const bool valid = t < w.n_count;
-	const uint32_t k = w.n_begin + (valid ? t : 0);
-	const uint2 ne = ent[k];
-	const uint32_t key = valid ? keys[k] : 0xFFFFFFFFu;
-	// step s of the segmented reduction adds lane + 2^s when it is in the
-	// same run of equal keys (runs are contiguous: the copy is sorted)
-	uint32_t add = 0;
+	extern __shared__ double tile[];
+	// tiles are sorted heaviest first and dispatched in that order
+	const TileWork w = work[blockIdx.x];
+	(void)nwork;
+	const int nrow = w.po_end - w.po_begin;
+	const int nn = w.nn_out;
+	const int telems = nrow * nn;
+	for (int t = threadIdx.x; t < telems; t += blockDim.x)
+		tile[t] = 0.0;
+	__syncthreads();
+	// tasks are taken by waves independently; all accumulation into the
+	// tile is by LDS atomics
+	const int lane = threadIdx.x & 63;
+	// wave-uniform by construction; readfirstlane tells the compiler, so
+	// the task, the instruction and all address bases live in SGPRs
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int nwaves = kTileThreads >> 6;
+	for (int32_t ti = w.t_begin + wave; ti < w.t_end; ti += nwaves) {
+		const Task T = tasks[ti];
+		const TInstr I = ins[T.vi];
+		const acc_t sc = acc_t(I.force == 1 ? s1 : (I.force == 2 ? s2 : s3));
+		if (I.type == 3) {
+			// proton entries [jb, je) of the tile's rows (contiguous in the
+			// copy ordered by row; keys[] gives each entry's row): lanes walk
+			// neighbouring neutron entries, i.e. neighbouring columns of the
+			// dense sub-block, two chunks of 64 at a time.  Proton entries
+			// arrive with one coalesced load per 64 and are broadcast with
+			// readlane; a lane's partial sum goes to the tile whenever the
+			// (uniform) row changes.
+			const VT *vblk = V + I.v_off;
+			const auto *xblk = x + I.x_off;
+			const uint32_t dn = (uint32_t)I.dn, nn_in = (uint32_t)I.nn_in;
+			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 128) {
+				const uint32_t eA = e0 + lane, eB = e0 + 64 + lane;
+				const bool vA = eA < T.e1, vB = eB < T.e1;
+				const uint32_t kA = I.n_base + (vA ? eA : T.e0);
+				const uint32_t kB = I.n_base + (vB ? eB : T.e0);
+				const uint2 nA = ent[kA], nB = ent[kB];
+				const uint32_t cA = nA.y & kIndexMask, cB = nB.y & kIndexMask;
+				const uint32_t oA = keys[kA], oB = keys[kB];
+				const acc_t sA = apply_sign(sc, nA.y), sB = apply_sign(sc, nB.y);
+				acc_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+				int cur = -1;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					const uint2 pl = ent[jl];
+					const uint32_t prow = keys[jl];
+					uint32_t t = 0;
+					while (t < cnt) {
+						const int row = (int)__builtin_amdgcn_readlane(prow, t);
+						if (row != cur) {
+							if (cur >= 0) {
+								const int r = cur - w.po_begin;
+								if (vA)
+									atomicAdd(&tile[r * nn + oA], (double)(sA * (a0 + a1)));
+								if (vB)
+									atomicAdd(&tile[r * nn + oB], (double)(sB * (b0 + b1)));
+							}
+							a0 = a1 = b0 = b1 = 0;
+							cur = row;
+						}
+						const bool pair = t + 1 < cnt &&
+							(int)__builtin_amdgcn_readlane(prow, t + 1) == row;
+						if (pair) {
+							acc_t va[2], vb[2], xa[2], xb[2];
 #pragma unroll
-	for (int s = 0; s < 6; s++) {
-		const uint32_t other = __shfl_down(key, 1 << s, 64);
-		if (lane + (1 << s) < 64 && other == key)
-			add |= 1u << s;
+							for (int u = 0; u < 2; u++) {
+								const uint32_t px = __builtin_amdgcn_readlane(pl.x, t + u);
+								const uint32_t py = __builtin_amdgcn_readlane(pl.y, t + u);
+								const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+								const auto *xcol = xblk + (size_t)(nn_in * px);
+								va[u] = apply_sign(Elem<VT>::load(vrow, cA), py);
+								vb[u] = apply_sign(Elem<VT>::load(vrow, cB), py);
+								xa[u] = acc_t(xcol[nA.x]);
+								xb[u] = acc_t(xcol[nB.x]);
+							}
+							a0 += va[0] * xa[0];
+							a1 += va[1] * xa[1];
+							b0 += vb[0] * xb[0];
+							b1 += vb[1] * xb[1];
+							t += 2;
+						} else {
+							const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+							const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+							const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+							const auto *xcol = xblk + (size_t)(nn_in * px);
+							a0 += apply_sign(Elem<VT>::load(vrow, cA), py) * acc_t(xcol[nA.x]);
+							b0 += apply_sign(Elem<VT>::load(vrow, cB), py) * acc_t(xcol[nB.x]);
+							t += 1;
+						}
+					}
+				}
+				if (cur >= 0) {
+					const int r = cur - w.po_begin;
+					if (vA)
+						atomicAdd(&tile[r * nn + oA], (double)(sA * (a0 + a1)));
+					if (vB)
+						atomicAdd(&tile[r * nn + oB], (double)(sB * (b0 + b1)));
+				}
+			}
+		} else if (I.type == 2) {
+			// proton entries [jb, je) of the tile's rows: lanes over
+			// neighbouring neutron states n (coalesced x); entries broadcast
+			// with readlane; partial sums go to the tile on row changes
+			const auto *xblk = x + I.x_off;
+			const uint32_t nn_in = (uint32_t)I.nn_in;
+			for (int n0 = (int)T.e0; n0 < (int)T.e1; n0 += 64) {
+				const int n = n0 + lane;
+				const bool valid = n < (int)T.e1;
+				const uint32_t nl = valid ? (uint32_t)n : T.e0;
+				acc_t a0 = 0, a1 = 0;
+				int cur = -1;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					const uint2 pl = ent[jl];
+					const uint32_t prow = keys[jl];
+					for (uint32_t t = 0; t < cnt; t++) {
+						const int row = (int)__builtin_amdgcn_readlane(prow, t);
+						if (row != cur) {
+							if (cur >= 0 && valid && (a0 + a1) != acc_t(0))
+								atomicAdd(&tile[(cur - w.po_begin) * nn + n], (double)(sc * (a0 + a1)));
+							a0 = a1 = 0;
+							cur = row;
+						}
+						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+						const acc_t e = Elem<VT>::load(V, I.v_off + (py & kIndexMask));
+						const acc_t xv = acc_t((xblk + (size_t)(nn_in * px))[nl]);
+						if (!(fabs(e) < acc_t(1e-12))) {
+							if (t & 1)
+								a1 += apply_sign(e, py) * xv;
+							else
+								a0 += apply_sign(e, py) * xv;
+						}
+					}
+				}
+				if (cur >= 0 && valid && (a0 + a1) != acc_t(0))
+					atomicAdd(&tile[(cur - w.po_begin) * nn + n], (double)(sc * (a0 + a1)));
+			}
+		} else {
+			// entries [jb, je) of the neutron list copy ordered by output
+			// row: lanes over entries, each applied to every row of the
+			// tile (the protons are spectators)
+			const auto *xblk = x + I.x_off;
+			const uint32_t nn_in = (uint32_t)I.nn_in;
+			for (uint32_t k0 = T.jb; k0 < T.je; k0 += 64) {
+				const uint32_t k = k0 + lane;
+				const bool valid = k < T.je;
+				const uint32_t kk = valid ? k : T.jb;
+				const uint2 en = ent[kk];
+				acc_t v = Elem<VT>::load(V, I.v_off + (en.y & kIndexMask));
+				if (!valid || fabs(v) < acc_t(1e-12))
+					continue;
+				v = apply_sign(v, en.y) * sc;
+				const uint32_t no = keys[kk];
+				const auto *xc = xblk + en.x + (size_t)nn_in * w.po_begin;
+				for (int r = 0; r < nrow; r++)
+					atomicAdd(&tile[r * nn + no], (double)(v * acc_t(xc[(size_t)nn_in * r])));
+			}
+		}
 	}
-	const uint32_t prev = __shfl_up(key, 1, 64);
-	const bool head = valid && (lane == 0 || prev != key);
-	const acc_t sc = apply_sign(acc_t(w.force == 1 ? s1 : (w.force == 2 ? s2 : s3)), ne.y);
-	const VT *vrow0 = V + w.v_off + (ne.y & kIndexMask);
-	const auto *xcol0 = x + w.x_off + ne.x;
-	double *ycol0 = y + w.y_off + key;
-	for (int po = w.po_begin; po < w.po_end; po++) {
-		const uint32_t jb = rowptr[w.p_rows + po];
-		const uint32_t je = rowptr[w.p_rows + po + 1];
-		if (jb == je)
-			continue;
-		acc_t acc = 0;
-		for (uint32_t j = jb; j < je; j++) {
-			const uint2 pe = ent[j];
-			const acc_t v = Elem<VT>::load(vrow0, w.dn * (int64_t)(pe.y & kIndexMask));
-			const acc_t xv = acc_t(xcol0[(int64_t)w.nn_in * pe.x]);
-			acc += apply_sign(v, pe.y) * xv;
-		}
-		acc *= sc;
-#pragma unroll
-		for (int s = 0; s < 6; s++) {
-			const acc_t o = __shfl_down(acc, 1 << s, 64);
-			if (add & (1u << s))
-				acc += o;
-		}
-		if (head)
-			atomicAdd(ycol0 + (int64_t)w.nn_out * po, (double)acc);
+	__syncthreads();
+	double *yblk = y + w.y_off + (int64_t)nn * w.po_begin;
+	for (int t = threadIdx.x; t < telems; t += blockDim.x) {
+		const double v = tile[t];
+		if (w.shared) {
+			if (v != 0.0)
+				atomicAdd(yblk + t, v);
+		} else
+			yblk[t] += v;
 	}
 }
 
@@ -237,14 +296,10 @@ template <typename VT>
 void launch(const bacchus_hip_operator *op, const VT *V, const typename Elem<VT>::x_t *x,
 	    double *y, double s1, double s2, double s3, hipStream_t s)
 {
-	if (op->num_npwork)
-		hipLaunchKernelGGL(np_kernel<VT>, dim3(op->num_npwork), dim3(kChunk), 0, s,
-				   op->d_npwork, op->num_npwork, op->d_rowptr, op->d_entries,
-				   op->d_keys, V, x, y, s1, s2, s3);
-	if (op->num_work)
-		hipLaunchKernelGGL(matvec_kernel<VT>, dim3(op->num_work), dim3(kChunk), 0, s,
-				   op->d_work, op->d_vinstr, op->d_rowptr, op->d_entries, V, x, y,
-				   s1, s2, s3);
+	if (op->num_tiles)
+		hipLaunchKernelGGL(tile_kernel<VT>, dim3(op->num_tiles), dim3(kTileThreads),
+				   op->tile_lds_bytes, s, op->d_tiles, op->num_tiles, op->d_tasks, op->d_tinstr,
+				   op->d_rowptr, op->d_entries, op->d_keys, V, x, y, s1, s2, s3);
 }
 
 uint16_t to_bf16(double v)
@@ -299,10 +354,10 @@ void bacchus_hip_operator_free(bacchus_hip_operator_t *op)
 	(void)hipFree(op->d_v);
 	(void)hipFree(op->d_rowptr);
 	(void)hipFree(op->d_entries);
-	(void)hipFree(op->d_vinstr);
-	(void)hipFree(op->d_work);
+	(void)hipFree(op->d_tinstr);
+	(void)hipFree(op->d_tiles);
+	(void)hipFree(op->d_tasks);
 	(void)hipFree(op->d_keys);
-	(void)hipFree(op->d_npwork);
 	(void)hipFree(op->d_xf);
 	delete op;
 }
@@ -329,7 +384,7 @@ int bacchus_hip_operator_stats(const bacchus_hip_operator_t *op,
 	if (flops)
 		*flops = op->flops;
 	if (num_work_items)
-		*num_work_items = op->num_work;
+		*num_work_items = op->num_tiles;
 	return 0;
 }
 
@@ -470,9 +525,62 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 						 (long long)use[l].n_out));
 	}
 
-	// --- CSR copies of every used list: by output and by input index,
-	// each row ordered by matrix index (neighbouring lanes then read
-	// neighbouring elements); keys[k] = row of entry k ---
+	// --- compact element indices: every used list gets the sorted set of
+	// element indices it references; entries carry the position in that
+	// set (phase bit kept).  Each (element block, neutron list, proton list)
+	// then gets a dense sub-block W: N/P = the referenced elements, NP =
+	// rows over the proton set, columns over the neutron set.  An
+	// instruction's element loads are then neighbours and every loaded line
+	// is used. ---
+	std::vector<std::vector<uint32_t>> mvals(nl);
+	for (int64_t l = 0; l < nl; l++) {
+		if (!use[l].used)
+			continue;
+		auto &mv = mvals[l];
+		for (int64_t k = p->list_offsets[l]; k < p->list_offsets[l + 1]; k++)
+			mv.push_back((uint32_t)p->list_entries[3 * k + 2] & 0x7FFFFFFFu);
+		std::sort(mv.begin(), mv.end());
+		mv.erase(std::unique(mv.begin(), mv.end()), mv.end());
+	}
+	auto compact = [&](int64_t l, int32_t m) -> uint32_t {
+		const uint32_t v = (uint32_t)m & 0x7FFFFFFFu;
+		const auto &mv = mvals[l];
+		const uint32_t c = (uint32_t)(std::lower_bound(mv.begin(), mv.end(), v) - mv.begin());
+		return c | ((uint32_t)m & 0x80000000u);
+	};
+	std::map<std::tuple<int32_t, int32_t, int32_t>, int64_t> sub_of;
+	std::vector<int64_t> sub_off(ni, -1);
+	std::vector<double> W;
+	for (int64_t i = 0; i < ni; i++) {
+		const bacchus_hip_instruction_t &I = p->instructions[i];
+		const auto key = std::make_tuple(I.matrix_block, I.type != 2 ? I.neutron_list : -1,
+						 I.type != 1 ? I.proton_list : -1);
+		auto it = sub_of.find(key);
+		if (it != sub_of.end()) {
+			sub_off[i] = it->second;
+			continue;
+		}
+		const int64_t off = (int64_t)W.size();
+		const double *Vb = p->matrix_elements + p->matrix_offsets[I.matrix_block];
+		if (I.type == 1 || I.type == 2) {
+			for (uint32_t m : mvals[I.type == 1 ? I.neutron_list : I.proton_list])
+				W.push_back(Vb[m]);
+		} else {
+			const auto &mn = mvals[I.neutron_list], &mp = mvals[I.proton_list];
+			const int64_t Dn = vdn[I.matrix_block];
+			W.resize(off + (int64_t)mn.size() * (int64_t)mp.size());
+			double *dst = W.data() + off;
+			for (size_t b = 0; b < mp.size(); b++)
+				for (size_t a2 = 0; a2 < mn.size(); a2++)
+					dst[b * mn.size() + a2] = Vb[mn[a2] + Dn * (int64_t)mp[b]];
+		}
+		sub_of[key] = off;
+		sub_off[i] = off;
+	}
+	op->compact_elements = (int64_t)W.size();
+
+	// --- CSR copies of every used list, by output and by input index,
+	// each row ordered by element index; keys[k] = row of entry k ---
 	std::vector<uint32_t> rowptr;
 	std::vector<uint2> entries;
 	std::vector<uint32_t> keys;
@@ -503,7 +611,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				const uint32_t r = (uint32_t)e3[key];
 				uint2 en;
 				en.x = (uint32_t)e3[1 - key];
-				en.y = (uint32_t)e3[2];
+				en.y = compact(l, e3[2]);
 				keys[base + fill[r]] = r;
 				entries[base + fill[r]++] = en;
 			}
@@ -518,47 +626,37 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		}
 	}
 	op->list_entry_count = used_entries;
-	// Optional ordering of the neutron lists of NP instructions by element
-	// index alone (coalesced V rows, shorter reduction runs); rows stay in
-	// keys[] so the NP kernel's segmented reduction is unchanged.
-	const char *order_env = getenv("BACCHUS_HIP_NP_ORDER");
-	const bool element_order = order_env && strcmp(order_env, "element") == 0;
+	// neutron lists of NP instructions: a copy ordered by element index
+	// alone (rows kept in keys[])
 	std::vector<int64_t> elem_by_out(nl, -1), elem_by_in(nl, -1);
-	if (element_order) {
-		for (int64_t i = 0; i < ni; i++) {
-			const bacchus_hip_instruction_t &I = p->instructions[i];
-			if (I.type != 3 || elem_by_out[I.neutron_list] >= 0)
-				continue;
-			const int64_t l = I.neutron_list;
-			for (int dir = 0; dir < 2; dir++) {
-				const int64_t r0 = (dir == 0 ? rows_by_out : rows_by_in)[l];
-				const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
-				const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
-				std::vector<uint32_t> perm(b - a);
-				std::iota(perm.begin(), perm.end(), a);
-				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
-					return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
-				});
-				(dir == 0 ? elem_by_out : elem_by_in)[l] = (int64_t)entries.size();
-				for (uint32_t k : perm) {
-					entries.push_back(entries[k]);
-					keys.push_back(keys[k]);
-				}
+	for (int64_t i = 0; i < ni; i++) {
+		const bacchus_hip_instruction_t &I = p->instructions[i];
+		if (I.type != 3 || elem_by_out[I.neutron_list] >= 0)
+			continue;
+		const int64_t l = I.neutron_list;
+		for (int dir = 0; dir < 2; dir++) {
+			const int64_t r0 = (dir == 0 ? rows_by_out : rows_by_in)[l];
+			const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
+			const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
+			std::vector<uint32_t> perm(b - a);
+			std::iota(perm.begin(), perm.end(), a);
+			std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+				return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
+			});
+			(dir == 0 ? elem_by_out : elem_by_in)[l] = (int64_t)entries.size();
+			for (uint32_t k : perm) {
+				const uint2 e = entries[k];
+				const uint32_t r = keys[k];
+				entries.push_back(e);
+				keys.push_back(r);
 			}
 		}
 		if (entries.size() > 0xFFFFFFFFull)
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
 	}
 
-	// --- virtual instructions: N and P grouped by output block; NP split
-	// into (neutron-entry chunk x proton output-row range) work items ---
-	std::vector<std::vector<VInstr>> per_block(nb);
-	std::vector<std::vector<double>> per_block_cost(nb);
-	struct NPV {
-		NPWork proto;
-		int64_t n_base, n_len, p_rows, n_pout;
-	};
-	std::vector<NPV> npv;
+	// --- virtual instructions (each direction) grouped by output block ---
+	std::vector<std::vector<TInstr>> per_block(nb);
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		const int force = p->matrix_force ? p->matrix_force[I.matrix_block] : 2;
@@ -566,52 +664,33 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 								 p->list_offsets[I.neutron_list] : 0;
 		const int64_t lp = I.proton_list >= 0 ? p->list_offsets[I.proton_list + 1] -
 								p->list_offsets[I.proton_list] : 0;
+		if ((I.type != 2 && ln == 0) || (I.type != 1 && lp == 0))
+			continue;
 		for (int dir = 0; dir < (I.block_in == I.block_out ? 1 : 2); dir++) {
 			const int64_t bin = dir == 0 ? I.block_in : I.block_out;
 			const int64_t bout = dir == 0 ? I.block_out : I.block_in;
 			const auto &rows = dir == 0 ? rows_by_out : rows_by_in;
-			const int f = std::min(3, std::max(1, force));
+			TInstr t{};
+			t.x_off = op->block_off[bin];
+			t.v_off = sub_off[i];
+			t.dn = I.type == 3 ? (int64_t)mvals[I.neutron_list].size() : 0;
+			t.nn_in = (int32_t)op->block_nn[bin];
+			t.type = (int16_t)I.type;
+			t.force = (int16_t)std::min(3, std::max(1, force));
 			if (I.type == 3) {
-				if (ln == 0 || lp == 0)
-					continue;
-				NPV q{};
-				q.proto.y_off = op->block_off[bout];
-				q.proto.x_off = op->block_off[bin];
-				q.proto.v_off = p->matrix_offsets[I.matrix_block];
-				q.proto.dn = vdn[I.matrix_block];
-				q.proto.nn_in = (int32_t)op->block_nn[bin];
-				q.proto.nn_out = (int32_t)op->block_nn[bout];
-				q.proto.force = f;
-				const int64_t nr = rows[I.neutron_list];
-				q.n_base = element_order
-					? (dir == 0 ? elem_by_out : elem_by_in)[I.neutron_list]
-					: (int64_t)rowptr[nr];
-				q.n_len = ln;
-				q.p_rows = rows[I.proton_list];
-				q.n_pout = op->block_np[bout];
-				npv.push_back(q);
-				continue;
-			}
-			VInstr v{};
-			v.x_off = op->block_off[bin];
-			v.v_off = p->matrix_offsets[I.matrix_block];
-			v.dn = 0;
-			v.nn_in = (int32_t)op->block_nn[bin];
-			v.type = (int16_t)I.type;
-			v.force = (int16_t)f;
-			if (I.type == 1)
-				v.n_rows = (uint32_t)rows[I.neutron_list];
-			else
-				v.p_rows = (uint32_t)rows[I.proton_list];
-			const double m = I.type == 1 ? double(ln) * op->block_np[bin]
-						     : double(lp) * op->block_nn[bin];
-			per_block[bout].push_back(v);
-			per_block_cost[bout].push_back(m);
+				t.n_base = (uint32_t)(dir == 0 ? elem_by_out : elem_by_in)[I.neutron_list];
+				t.n_len = (uint32_t)ln;
+			} else if (I.type == 1)
+				t.n_base = (uint32_t)rows[I.neutron_list];
+			if (I.type != 1)
+				t.p_rows = (uint32_t)rows[I.proton_list];
+			per_block[bout].push_back(t);
 		}
 	}
-	// --- N/P work items: 256-element chunks x instruction ranges ---
-	std::vector<VInstr> vinstr;
-	std::vector<WorkItem> work;
+
+	// --- tiles: rows of a block sized to ~target multiplications and the
+	// LDS budget; a single overloaded row is split over instruction ranges
+	// (those tiles flush with atomics) ---
 	int num_cu = 256;
 	{
 		int dev = 0;
@@ -620,92 +699,175 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		    hipGetDeviceProperties(&prop, dev) == hipSuccess)
 			num_cu = prop.multiProcessorCount;
 	}
-	double np_mults = 0;
-	for (auto &c : per_block_cost)
-		for (double v : c)
-			np_mults += v;
-	const double target = std::max(2.0e4, np_mults / (num_cu * 32.0));
+	// ~64 tiles per CU: small tiles keep every CU busy to the end
+	double target = std::max(2.0e4, mults / (num_cu * 64.0));
+	if (const char *tm = getenv("BACCHUS_HIP_TILE_MULTS"))
+		target = std::max(1.0e3, atof(tm));
+	std::vector<TInstr> tinstr;
+	std::vector<TileWork> tiles;
+	int64_t max_tile = 1;
 	for (int64_t b = 0; b < nb; b++) {
 		auto &vs = per_block[b];
-		auto &cs = per_block_cost[b];
 		if (vs.empty())
 			continue;
-		std::vector<int64_t> order(vs.size());
-		std::iota(order.begin(), order.end(), 0);
-		std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t c) {
-			return vs[a].v_off < vs[c].v_off;
+		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
+			return u.v_off < w.v_off;
 		});
-		const int32_t first = (int32_t)vinstr.size();
-		std::vector<double> cost;
-		for (int64_t k : order) {
-			vinstr.push_back(vs[k]);
-			cost.push_back(cs[k]);
-		}
-		const int64_t size = op->block_np[b] * op->block_nn[b];
-		for (int64_t e0 = 0; e0 < size; e0 += kChunk) {
-			const int32_t cnt = (int32_t)std::min<int64_t>(kChunk, size - e0);
-			const double frac = double(cnt) / double(size);
-			int32_t begin = 0;
-			double acc = 0;
-			for (int32_t k = 0; k < (int32_t)cost.size(); k++) {
-				acc += cost[k] * frac + 64.0;
-				if (acc >= target || k + 1 == (int32_t)cost.size()) {
-					WorkItem w{};
-					w.y_off = op->block_off[b] + e0;
-					w.count = cnt;
-					w.nn_out = (int32_t)op->block_nn[b];
-					w.elem0 = (int32_t)e0;
-					w.vi_begin = first + begin;
-					w.vi_end = first + k + 1;
-					work.push_back(w);
-					begin = k + 1;
-					acc = 0;
-				}
+		const int32_t first = (int32_t)tinstr.size();
+		tinstr.insert(tinstr.end(), vs.begin(), vs.end());
+		const int64_t nn = op->block_nn[b], np = op->block_np[b];
+		if (nn > kTileMaxElems)
+			return fail(BACCHUS_FAIL("basis block %lld has %lld neutron states; the tile "
+						 "kernel holds at most %d", (long long)b, (long long)nn,
+						 kTileMaxElems));
+		// cost of every (instruction, row) and of every row
+		std::vector<std::vector<double>> cost(vs.size());
+		std::vector<double> row_cost(np, 0.0);
+		for (size_t k = 0; k < vs.size(); k++) {
+			const TInstr &t = vs[k];
+			cost[k].assign(np, 0.0);
+			for (int64_t po = 0; po < np; po++) {
+				double c;
+				if (t.type == 3)
+					c = double(t.n_len) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
+					    t.n_len * 0.25;
+				else if (t.type == 1)
+					c = double(rowptr[t.n_base + nn] - rowptr[t.n_base]) + nn * 0.25;
+				else
+					c = double(nn) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
+					    nn * 0.25;
+				cost[k][po] = c;
+				row_cost[po] += c;
 			}
 		}
-	}
-	op->num_work = (int64_t)work.size();
-	op->num_vinstr = (int64_t)vinstr.size();
-
-	// --- NP work items, grouped by element block (V locality), then
-	// proton output-row range, then neutron chunk ---
-	std::stable_sort(npv.begin(), npv.end(), [](const NPV &a, const NPV &b) {
-		return a.proto.v_off < b.proto.v_off;
-	});
-	std::vector<NPWork> npwork;
-	for (const NPV &q : npv) {
-		// proton output rows split so a work item sees <= kProtonRange
-		// proton entries (a single long row stays whole)
-		int32_t po0 = 0;
-		while (po0 < q.n_pout) {
-			int32_t po1 = po0;
-			uint32_t taken = 0;
-			while (po1 < q.n_pout) {
-				const uint32_t len = rowptr[q.p_rows + po1 + 1] - rowptr[q.p_rows + po1];
-				if (po1 > po0 && taken + len > kProtonRange)
-					break;
-				taken += len;
+		const int64_t rows_max = std::max<int64_t>(1, kTileMaxElems / nn);
+		int64_t po0 = 0;
+		while (po0 < np) {
+			int64_t po1 = po0 + 1;
+			double c = row_cost[po0];
+			while (po1 < np && po1 - po0 < rows_max && c + row_cost[po1] <= target) {
+				c += row_cost[po1];
 				po1++;
 			}
-			if (taken > 0) {
-				for (int64_t c = 0; c < q.n_len; c += kChunk) {
-					NPWork w = q.proto;
-					w.n_begin = (uint32_t)(q.n_base + c);
-					w.n_count = (int32_t)std::min<int64_t>(kChunk, q.n_len - c);
-					w.p_rows = (uint32_t)q.p_rows;
-					w.po_begin = po0;
-					w.po_end = po1;
-					npwork.push_back(w);
+			max_tile = std::max(max_tile, (po1 - po0) * nn);
+			const int pieces = c > 2.0 * target ? (int)std::ceil(c / target) : 1;
+			if (pieces == 1) {
+				TileWork w{};
+				w.y_off = op->block_off[b];
+				w.nn_out = (int32_t)nn;
+				w.po_begin = (int32_t)po0;
+				w.po_end = (int32_t)po1;
+				w.vi_begin = first;
+				w.vi_end = first + (int32_t)vs.size();
+				w.shared = 0;
+				tiles.push_back(w);
+			} else {
+				const double per = c / pieces;
+				int32_t begin = 0;
+				double acc = 0;
+				for (size_t k = 0; k < vs.size(); k++) {
+					for (int64_t po = po0; po < po1; po++)
+						acc += cost[k][po];
+					if (acc >= per || k + 1 == vs.size()) {
+						TileWork w{};
+						w.y_off = op->block_off[b];
+						w.nn_out = (int32_t)nn;
+						w.po_begin = (int32_t)po0;
+						w.po_end = (int32_t)po1;
+						w.vi_begin = first + begin;
+						w.vi_end = first + (int32_t)k + 1;
+						w.shared = 1;
+						tiles.push_back(w);
+						begin = (int32_t)k + 1;
+						acc = 0;
+					}
 				}
 			}
 			po0 = po1;
 		}
 	}
-	// pad to a multiple of the XCD count so the XCD-aware block remap
-	// is a bijection (empty items do nothing)
-	while (npwork.size() % kXcds)
-		npwork.push_back(NPWork{});
-	op->num_npwork = (int64_t)npwork.size();
+	// heaviest tiles first, so the long ones do not form the tail
+	{
+		std::vector<double> tcost(tiles.size(), 0.0);
+		for (size_t ti = 0; ti < tiles.size(); ti++) {
+			const TileWork &t = tiles[ti];
+			for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
+				const TInstr &I = tinstr[v];
+				for (int32_t po = t.po_begin; po < t.po_end; po++) {
+					const double seg = I.type != 1 ? double(rowptr[I.p_rows + po + 1] - rowptr[I.p_rows + po]) : 0;
+					tcost[ti] += I.type == 3 ? I.n_len * seg
+						     : (I.type == 2 ? t.nn_out * seg
+								    : double(rowptr[I.n_base + t.nn_out] - rowptr[I.n_base]));
+				}
+			}
+		}
+		std::vector<size_t> order(tiles.size());
+		std::iota(order.begin(), order.end(), 0);
+		std::stable_sort(order.begin(), order.end(),
+				 [&](size_t a2, size_t b2) { return tcost[a2] > tcost[b2]; });
+		std::vector<TileWork> sorted;
+		for (size_t k : order)
+			sorted.push_back(tiles[k]);
+		tiles.swap(sorted);
+	}
+	// tasks of every tile: N instructions in entry ranges (all rows), P and
+	// NP once per row with a non-empty proton segment
+	std::vector<Task> tasks;
+	for (TileWork &t : tiles) {
+		t.t_begin = (int32_t)tasks.size();
+		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
+			const TInstr &I = tinstr[v];
+			if (I.type == 1) {
+				const uint32_t b = rowptr[I.n_base], e = rowptr[I.n_base + t.nn_out];
+				for (uint32_t k = b; k < e; k += kNTaskEntries)
+					tasks.push_back(Task{v, -1, k, std::min(e, k + kNTaskEntries), 0, 0, 0, 0});
+				continue;
+			}
+			// P and NP: the proton entries of the tile's rows form one
+			// contiguous segment; tasks cut it (and the neutron range) so
+			// each holds about kTaskMults multiplications
+			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
+			if (je <= jb)
+				continue;
+			const uint32_t span = I.type == 3 ? I.n_len : (uint32_t)t.nn_out;
+			const uint32_t unit = I.type == 3 ? 128u : 64u;
+			const uint32_t e_step = std::min<uint32_t>((span + unit - 1) / unit * unit, 1024u);
+			const uint32_t j_step = std::max<uint32_t>(64u, (uint32_t)(kTaskMults / e_step));
+			for (uint32_t jj = jb; jj < je; jj += j_step)
+				for (uint32_t e = 0; e < span; e += e_step)
+					tasks.push_back(Task{v, 0, jj, std::min(je, jj + j_step), e,
+							     std::min(span, e + e_step), 0, 0});
+		}
+		t.t_end = (int32_t)tasks.size();
+		if (tasks.size() > 0x7FFFFFFFull)
+			return fail(BACCHUS_FAIL("more than 2^31 tile tasks"));
+	}
+	op->num_tasks = (int64_t)tasks.size();
+	if (getenv("BACCHUS_HIP_VERBOSE")) {
+		// tile cost summary (multiplications per tile) for tuning
+		std::vector<double> tc;
+		for (const TileWork &t : tiles) {
+			double c = 0;
+			for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
+				const TInstr &I = tinstr[v];
+				for (int32_t po = t.po_begin; po < t.po_end; po++) {
+					const double seg = I.type != 1 ? double(rowptr[I.p_rows + po + 1] - rowptr[I.p_rows + po]) : 0;
+					c += I.type == 3 ? I.n_len * seg : (I.type == 2 ? t.nn_out * seg : double(rowptr[I.n_base + t.nn_out] - rowptr[I.n_base]));
+				}
+			}
+			tc.push_back(c);
+		}
+		std::sort(tc.begin(), tc.end());
+		double sum = 0;
+		for (double c : tc)
+			sum += c;
+		fprintf(stderr, "tiles %zu tasks %zu target %.3g sum %.4g mean %.3g p50 %.3g p90 %.3g p99 %.3g max %.3g\n",
+			tc.size(), tasks.size(), target, sum, sum / tc.size(), tc[tc.size() / 2],
+			tc[tc.size() * 9 / 10], tc[tc.size() * 99 / 100], tc.back());
+	}
+	op->num_tiles = (int64_t)tiles.size();
+	op->num_tinstr = (int64_t)tinstr.size();
+	op->tile_lds_bytes = (size_t)max_tile * sizeof(double);
 
 	// --- statistics: compulsory bytes of referenced arrays ---
 	std::vector<char> vref(nv, 0);
@@ -722,7 +884,6 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	op->flops = 2.0 * mults;
 
 	// --- upload ---
-	const int64_t total_v = nv ? p->matrix_offsets[nv] : 0;
 	auto upload = [&](void **dst, const void *src, size_t bytes) -> int {
 		if (bytes == 0)
 			bytes = 16;
@@ -731,18 +892,19 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			BACCHUS_HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
 		return 0;
 	};
+	const int64_t total_v = (int64_t)W.size();
 	int rc = 0;
 	if (precision == BACCHUS_HIP_FP64) {
-		rc = upload(&op->d_v, p->matrix_elements, total_v * sizeof(double));
+		rc = upload(&op->d_v, W.data(), total_v * sizeof(double));
 	} else if (precision == BACCHUS_HIP_FP32) {
 		std::vector<float> tmp(total_v);
 		for (int64_t k = 0; k < total_v; k++)
-			tmp[k] = (float)p->matrix_elements[k];
+			tmp[k] = (float)W[k];
 		rc = upload(&op->d_v, tmp.data(), total_v * sizeof(float));
 	} else {
 		std::vector<uint16_t> tmp(total_v);
 		for (int64_t k = 0; k < total_v; k++)
-			tmp[k] = to_bf16(p->matrix_elements[k]);
+			tmp[k] = to_bf16(W[k]);
 		rc = upload(&op->d_v, tmp.data(), total_v * sizeof(uint16_t));
 	}
 	if (rc)
@@ -750,9 +912,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	if ((rc = upload((void **)&op->d_rowptr, rowptr.data(), rowptr.size() * 4)) ||
 	    (rc = upload((void **)&op->d_entries, entries.data(), entries.size() * 8)) ||
 	    (rc = upload((void **)&op->d_keys, keys.data(), keys.size() * 4)) ||
-	    (rc = upload((void **)&op->d_npwork, npwork.data(), npwork.size() * sizeof(NPWork))) ||
-	    (rc = upload((void **)&op->d_vinstr, vinstr.data(), vinstr.size() * sizeof(VInstr))) ||
-	    (rc = upload((void **)&op->d_work, work.data(), work.size() * sizeof(WorkItem))))
+	    (rc = upload((void **)&op->d_tiles, tiles.data(), tiles.size() * sizeof(TileWork))) ||
+	    (rc = upload((void **)&op->d_tasks, tasks.data(), tasks.size() * sizeof(Task))) ||
+	    (rc = upload((void **)&op->d_tinstr, tinstr.data(), tinstr.size() * sizeof(TInstr))))
 		return fail(rc);
 	if (precision != BACCHUS_HIP_FP64 &&
 	    (rc = upload((void **)&op->d_xf, nullptr, op->dimension * sizeof(float))))
