@@ -158,6 +158,76 @@ int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 			double *ritz_vector,
 			bacchus_hip_lanczos_result_t *result);
 
+/* Batched form (no counterpart in the reference, where a parameter sweep is
+ * one Bacchus job per Hamiltonian after a Vulcan linear combination,
+ * src/Vulcan/programs/vulcan.c:114-207).  A batch holds several operators
+ * (basis sizes may differ) and B_i vectors per operator; vector b of
+ * operator i is multiplied by H_i(s) = s1 H1 + s2 V2 + s3 V3 with its own
+ * scales, the element blocks being shared by the whole batch.  One
+ * persistent launch covers every operator.  Operators must be FP32 or BF16
+ * (FP32 vectors, FP32 accumulation); they must outlive the batch.
+ * Vectors are device FP32 arrays [D_i][B_i], the vector index fastest. */
+typedef struct bacchus_hip_batch bacchus_hip_batch_t;
+
+typedef struct bacchus_hip_batch_tuning {
+	int32_t tiles_per_cu;     /* tile size: total work / (CUs * this); 0 = 16 */
+	int32_t max_tile_vectors; /* vectors per tile, power of two <= 64; 0 = 64 */
+	int32_t workgroups_per_cu; /* persistent grid; 0 = 8                    */
+	int32_t tile_words;       /* FP32 LDS words per tile, <= 16384; 0 = 8192 */
+} bacchus_hip_batch_tuning_t;
+
+int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops,
+			     const int32_t *num_vectors, int32_t num_ops,
+			     const bacchus_hip_batch_tuning_t *tuning,
+			     bacchus_hip_batch_t **batch);
+void bacchus_hip_batch_free(bacchus_hip_batch_t *batch);
+
+/* scales: num_vectors[op_index] x 3 doubles (s1, s2, s3 per vector) */
+int bacchus_hip_batch_set_scales(bacchus_hip_batch_t *batch, int32_t op_index,
+				 const double *scales);
+
+/* y_i = H_i x_i for every operator (x, y: num_ops device pointers; y is
+ * overwritten).  Pointers that change between calls are re-uploaded. */
+int bacchus_hip_batch_matvec(bacchus_hip_batch_t *batch, const float *const *x,
+			     float *const *y, void *stream);
+
+/* Times candidate tilings (repetitions matvecs each, on internal buffers)
+ * and keeps the fastest; writes it and its time per batched matvec. */
+int bacchus_hip_batch_autotune(bacchus_hip_batch_t *batch, int32_t repetitions,
+			       bacchus_hip_batch_tuning_t *best, double *best_ms);
+
+/* bytes = sum_i (s_V |V_i| + 12 |L_i| + 8 D_i B_i); flops = 2 sum_i
+ * mults_i B_i; num_items = work items of the persistent launch */
+int bacchus_hip_batch_stats(const bacchus_hip_batch_t *batch, double *bytes,
+			    double *flops, int64_t *num_items);
+
+/* One Lanczos recurrence per (operator, vector) in lockstep: full CGS2
+ * reorthogonalization and all dot products in FP64, the matvec batched.
+ * Each iteration is one captured hipGraph when use_graph != 0. */
+typedef struct bacchus_hip_batch_lanczos_settings {
+	int64_t max_num_iterations;
+	int32_t num_eigenvalues;   /* lowest eigenvalues per instance        */
+	int32_t use_graph;
+	double eigenvalue_tolerance; /* 0: run max_num_iterations; else stop
+					when every instance's lowest
+					num_eigenvalues moved less than this
+					(relative) between checks */
+	int32_t check_every;
+	int32_t pad;
+	uint64_t seed;             /* random starts                          */
+	const double *const *start_vectors; /* NULL or per operator NULL/device
+					       FP64 [D_i][B_i]             */
+} bacchus_hip_batch_lanczos_settings_t;
+
+/* eigenvalues: host, sum_i B_i * num_eigenvalues, operator-major then
+ * vector then ascending eigenvalue; ritz_vectors: NULL or per operator
+ * NULL/device FP64 [D_i][B_i] receiving each instance's lowest Ritz
+ * vector (unit norm). */
+int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *batch,
+			      const bacchus_hip_batch_lanczos_settings_t *settings,
+			      double *eigenvalues, double *const *ritz_vectors,
+			      bacchus_hip_lanczos_result_t *result);
+
 /* Learned Lanczos start (no counterpart in the reference): an MLP
  * in -> hidden (x depth, GELU) -> out with BF16 weights on MFMA, FP32
  * accumulation and FP32 master weights trained with Adam on the device.

@@ -45,6 +45,9 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_last_error", "bacchus_hip_mlp_create", "bacchus_hip_mlp_free",
     "bacchus_hip_mlp_num_params", "bacchus_hip_mlp_get_params", "bacchus_hip_mlp_set_params",
     "bacchus_hip_mlp_forward", "bacchus_hip_mlp_gradients", "bacchus_hip_mlp_train_step",
+    "bacchus_hip_batch_create", "bacchus_hip_batch_free", "bacchus_hip_batch_set_scales",
+    "bacchus_hip_batch_matvec", "bacchus_hip_batch_autotune", "bacchus_hip_batch_stats",
+    "bacchus_hip_batch_lanczos",
 )
 
 
@@ -73,6 +76,18 @@ class _LanczosSettings(ctypes.Structure):
         ("target_eigenvalues", ctypes.c_void_p), ("check_every", ctypes.c_int32),
         ("pad", ctypes.c_int32),
     ]
+
+
+class _BatchTuning(ctypes.Structure):
+    _fields_ = [("tiles_per_cu", ctypes.c_int32), ("max_tile_vectors", ctypes.c_int32),
+                ("workgroups_per_cu", ctypes.c_int32), ("tile_words", ctypes.c_int32)]
+
+
+class _BatchLanczosSettings(ctypes.Structure):
+    _fields_ = [("max_num_iterations", ctypes.c_int64), ("num_eigenvalues", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("eigenvalue_tolerance", ctypes.c_double),
+                ("check_every", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("start_vectors", ctypes.c_void_p)]
 
 
 class _LanczosResult(ctypes.Structure):
@@ -126,6 +141,24 @@ def library() -> ctypes.CDLL:
                                         ctypes.POINTER(_LanczosResult)]
     lib.bacchus_hip_last_error.restype = ctypes.c_char_p
     lib.bacchus_hip_last_error.argtypes = []
+    lib.bacchus_hip_batch_create.restype = c_int
+    lib.bacchus_hip_batch_create.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(_BatchTuning),
+                                             ctypes.POINTER(vp)]
+    lib.bacchus_hip_batch_free.restype = None
+    lib.bacchus_hip_batch_free.argtypes = [vp]
+    lib.bacchus_hip_batch_set_scales.restype = c_int
+    lib.bacchus_hip_batch_set_scales.argtypes = [vp, ctypes.c_int32, vp]
+    lib.bacchus_hip_batch_matvec.restype = c_int
+    lib.bacchus_hip_batch_matvec.argtypes = [vp, vp, vp, vp]
+    lib.bacchus_hip_batch_autotune.restype = c_int
+    lib.bacchus_hip_batch_autotune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(_BatchTuning),
+                                               ctypes.POINTER(ctypes.c_double)]
+    lib.bacchus_hip_batch_stats.restype = c_int
+    lib.bacchus_hip_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+    lib.bacchus_hip_batch_lanczos.restype = c_int
+    lib.bacchus_hip_batch_lanczos.argtypes = [vp, ctypes.POINTER(_BatchLanczosSettings), vp, vp,
+                                              ctypes.POINTER(_LanczosResult)]
     _LIB = lib
     return lib
 
@@ -298,6 +331,118 @@ def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
     k = int(res.iterations)
     return LanczosResult(ev[:min(k, num_eigenvalues)], alpha[:k], beta[:k], k,
                          res.matvec_seconds, res.total_seconds)
+
+
+class Batch:
+    """Several operators (FP32 or BF16 elements) with B_i vectors each, every
+    vector with its own (s1, s2, s3) scales, multiplied in one persistent
+    launch.  Vectors are float32 device tensors of shape (D_i, B_i)."""
+
+    def __init__(self, operators, num_vectors, tiles_per_cu=0, max_tile_vectors=0,
+                 workgroups_per_cu=0, tile_words=0):
+        lib = library()
+        self.operators = list(operators)
+        self.num_vectors = [int(b) for b in num_vectors]
+        if len(self.operators) != len(self.num_vectors):
+            raise BacchusError("one vector count per operator")
+        hs = (ctypes.c_void_p * len(self.operators))(*[o.handle.value for o in self.operators])
+        nv = np.ascontiguousarray(self.num_vectors, dtype=np.int32)
+        tu = _BatchTuning(tiles_per_cu, max_tile_vectors, workgroups_per_cu, tile_words)
+        h = ctypes.c_void_p()
+        _check(lib.bacchus_hip_batch_create(ctypes.cast(hs, ctypes.c_void_p), _ptr(nv),
+                                            len(self.operators), ctypes.byref(tu), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None or not self._h.value:
+            raise BacchusError("batch is closed")
+        return self._h
+
+    def set_scales(self, index, scales):
+        """scales: (B_index, 3) array of (s1, s2, s3) per vector."""
+        sc = np.ascontiguousarray(scales, dtype=np.float64).reshape(self.num_vectors[index], 3)
+        _check(library().bacchus_hip_batch_set_scales(self.handle, int(index), _ptr(sc)))
+
+    def matvec(self, xs, ys, stream=None):
+        import torch
+        for i, (x, y) in enumerate(zip(xs, ys)):
+            shape = (self.operators[i].dimension, self.num_vectors[i])
+            for t in (x, y):
+                if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() \
+                        or tuple(t.shape) != shape:
+                    raise BacchusError(f"operator {i}: vectors must be contiguous float32 "
+                                       f"device tensors of shape {shape}")
+        n = len(self.operators)
+        xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        yp = (ctypes.c_void_p * n)(*[y.data_ptr() for y in ys])
+        _check(library().bacchus_hip_batch_matvec(self.handle, ctypes.cast(xp, ctypes.c_void_p),
+                                                  ctypes.cast(yp, ctypes.c_void_p),
+                                                  _stream_handle(stream)))
+        return ys
+
+    def autotune(self, repetitions=3):
+        best = _BatchTuning()
+        ms = ctypes.c_double()
+        _check(library().bacchus_hip_batch_autotune(self.handle, int(repetitions),
+                                                    ctypes.byref(best), ctypes.byref(ms)))
+        return {"tiles_per_cu": best.tiles_per_cu, "max_tile_vectors": best.max_tile_vectors,
+                "workgroups_per_cu": best.workgroups_per_cu, "tile_words": best.tile_words,
+                "ms": ms.value}
+
+    def stats(self) -> dict:
+        b, f = ctypes.c_double(), ctypes.c_double()
+        w = ctypes.c_int64()
+        _check(library().bacchus_hip_batch_stats(self.handle, ctypes.byref(b), ctypes.byref(f),
+                                                 ctypes.byref(w)))
+        return {"algorithmic_bytes": b.value, "flops": f.value, "work_items": w.value}
+
+    def lanczos(self, max_num_iterations, num_eigenvalues=3, tolerance=0.0, check_every=5,
+                use_graph=True, seed=1, start_vectors=None, ritz_vectors=None):
+        """Independent Lanczos recurrences for every (operator, vector).
+        Returns (eigenvalues, iterations, seconds); eigenvalues is a list of
+        (B_i, num_eigenvalues) arrays.  start_vectors / ritz_vectors: lists
+        of float64 device tensors (D_i, B_i) or None."""
+        n = len(self.operators)
+        sv = None
+        if start_vectors is not None:
+            sv = (ctypes.c_void_p * n)(*[t.data_ptr() if t is not None else None
+                                         for t in start_vectors])
+        st = _BatchLanczosSettings(int(max_num_iterations), int(num_eigenvalues),
+                                   int(bool(use_graph)), float(tolerance), int(check_every), 0,
+                                   int(seed), ctypes.cast(sv, ctypes.c_void_p) if sv else None)
+        total = sum(self.num_vectors) * num_eigenvalues
+        ev = np.zeros(total)
+        rv = None
+        if ritz_vectors is not None:
+            rv = (ctypes.c_void_p * n)(*[t.data_ptr() if t is not None else None
+                                         for t in ritz_vectors])
+        res = _LanczosResult()
+        _check(library().bacchus_hip_batch_lanczos(self.handle, ctypes.byref(st), _ptr(ev),
+                                                   ctypes.cast(rv, ctypes.c_void_p) if rv else None,
+                                                   ctypes.byref(res)))
+        out, pos = [], 0
+        for b in self.num_vectors:
+            out.append(ev[pos:pos + b * num_eigenvalues].reshape(b, num_eigenvalues))
+            pos += b * num_eigenvalues
+        return out, int(res.iterations), float(res.total_seconds)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            library().bacchus_hip_batch_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class MLP:

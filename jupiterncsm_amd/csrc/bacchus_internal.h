@@ -78,7 +78,31 @@ constexpr int kTileThreads = 256;
 constexpr int kTileMaxElems = 2048;  // doubles of LDS per tile (16 KiB)
 constexpr uint32_t kNTaskEntries = 2048;  // list entries per N task
 constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
-constexpr int kXcds = 8;
+
+// Tiling of one operator's output: the tiles (rows of a block x all
+// neutron states x bt vectors) with their tasks, heaviest first.
+struct PlanParams {
+	double target = 0;          // multiplications per tile (x bt)
+	std::vector<int> bt;        // vectors per tile, per basis block
+	int64_t tile_elems = kTileMaxElems;  // LDS words per tile (rows*nn*bt)
+	uint64_t task_mults = kTaskMults;    // multiplications x bt per task
+};
+struct Plan {
+	std::vector<TileWork> tiles;
+	std::vector<Task> tasks;
+	std::vector<double> cost;   // multiplications x bt of every tile
+	std::vector<int> bt;        // vectors of every tile
+	int64_t max_tile_elems = 1;
+};
+
+// Host tridiagonal helpers (diag d, off-diagonal e, order k): lowest
+// `count` eigenvalues by Sturm bisection, and the eigenvector of one
+// eigenvalue by inverse iteration (component 0 >= 0).
+std::vector<double> tridiagonal_lowest_host(const double *d, const double *e,
+					    int64_t k, int count);
+std::vector<double> tridiagonal_vector(const std::vector<double> &d,
+				       const std::vector<double> &e, int64_t k,
+				       double lambda);
 
 }  // namespace bacchus
 
@@ -101,6 +125,13 @@ struct bacchus_hip_operator {
 	int64_t num_tiles = 0;
 	int64_t num_tinstr = 0;
 	size_t tile_lds_bytes = 0;
+	// host copies the planner needs: CSR row pointers and the virtual
+	// instructions grouped by output block ([blk_vi[b], blk_vi[b+1]))
+	std::vector<uint32_t> h_rowptr;
+	std::vector<bacchus::TInstr> h_tinstr;
+	std::vector<int32_t> blk_vi;
+	double mults = 0;
+	int num_cu = 256;
 	// statistics
 	double algorithmic_bytes = 0;
 	double flops = 0;
@@ -108,3 +139,7 @@ struct bacchus_hip_operator {
 	int64_t compact_elements = 0;  // elements of the dense sub-blocks
 	int64_t list_entry_count = 0;
 };
+
+namespace bacchus {
+int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out);
+}

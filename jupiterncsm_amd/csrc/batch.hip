@@ -1,0 +1,1132 @@
+// batch.hip - the batched form of the Bacchus matvec and Lanczos: several
+// operators, several vectors per operator, one persistent launch.
+//
+// A batch multiplies vector b of operator i by H_i(s_b) = s1 H1 + s2 V2 +
+// s3 V3 (the linear combination src/Vulcan/programs/vulcan.c:114-207
+// writes into new element files, applied here as per-vector scales of the
+// shared element blocks).  The work units are the owner-computes tiles of
+// the single-vector kernel (operator.hip) widened by bt vectors: the LDS
+// tile holds rows x neutron states x bt FP32 sums, lanes are (entry slot,
+// vector) pairs, so every element and list entry a lane group loads is used
+// for bt vectors and the input loads are bt contiguous floats.  Tiles of all
+// operators, times their vector chunks, form one list ordered heaviest
+// first that a persistent grid walks with a static stride.
+#include "bacchus_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace bacchus {
+namespace {
+
+constexpr uint32_t kMask = 0x7FFFFFFFu;
+constexpr int64_t kBatchTileWords = 16384;  // FP32 words of LDS per tile (64 KiB)
+constexpr int kMaxVectors = 256;
+
+struct BatchOp {
+	const TileWork *tiles;
+	const Task *tasks;
+	const TInstr *ins;
+	const uint2 *ent;
+	const uint32_t *keys;
+	const void *V;
+	const float *x;
+	float *y;
+	const float *scales;  // [B][4], index 1..3 = force class
+	int32_t B;
+	int32_t precision;
+};
+
+struct BatchItem {
+	int32_t op;
+	int32_t tile;
+	int32_t b0;
+	int32_t lbt;  // log2 of the vectors of the tile
+};
+
+__device__ __forceinline__ float sign_of(float v, uint32_t m)
+{
+	return __int_as_float(__float_as_int(v) ^ (int)(m & 0x80000000u));
+}
+
+template <typename VT> struct Load;
+template <> struct Load<float> {
+	__device__ static float at(const float *p, size_t i) { return p[i]; }
+};
+template <> struct Load<uint16_t> {
+	__device__ static float at(const uint16_t *p, size_t i)
+	{
+		return __uint_as_float(uint32_t(p[i]) << 16);
+	}
+};
+
+// One work item: tile w of operator o for vectors [b0, b0 + bt).
+//   NP (matrix_vector_multiplication.c:134-210): lanes = (neutron entry
+//     slot, vector); proton entries of the tile's rows are broadcast with
+//     readlane; sums go to the LDS tile when the proton row changes.
+//   P (:79-132): lanes = (neutron state slot, vector), same broadcast.
+//   N (:20-77): lanes = (entry slot, vector), each entry applied to every
+//     row of the tile.
+// N and P skip |V| < 1e-12 as the reference does (:52, :107).
+template <typename VT>
+__device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
+			   float *tile)
+{
+	const int bt = 1 << lbt;
+	const int nrow = w.po_end - w.po_begin;
+	const int nn = w.nn_out;
+	const int twords = (nrow * nn) << lbt;
+	for (int t = threadIdx.x; t < twords; t += blockDim.x)
+		tile[t] = 0.f;
+	__syncthreads();
+	const int lane = threadIdx.x & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int nwaves = blockDim.x >> 6;
+	const int slot = lane >> lbt;
+	const int G = 64 >> lbt;  // entry slots per pass
+	const int bl = lane & (bt - 1);
+	const int B = o.B;
+	const int b = b0 + bl;
+	const bool bvalid = b < B;
+	const VT *V = (const VT *)o.V;
+	for (int32_t ti = w.t_begin + wave; ti < w.t_end; ti += nwaves) {
+		const Task T = o.tasks[ti];
+		const TInstr I = o.ins[T.vi];
+		const float sc = bvalid ? o.scales[b * 4 + I.force] : 0.f;
+		// lane's column of the input block: element e at xb[e * B]
+		const float *xb = o.x + (size_t)I.x_off * B + (bvalid ? b : 0);
+		const uint32_t nn_in = (uint32_t)I.nn_in;
+		if (I.type == 3) {
+			const VT *vblk = V + I.v_off;
+			const uint32_t dn = (uint32_t)I.dn;
+			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 2 * G) {
+				const uint32_t eA = e0 + slot, eB = e0 + G + slot;
+				const bool vA = eA < T.e1 && bvalid, vB = eB < T.e1 && bvalid;
+				const uint32_t kA = I.n_base + (eA < T.e1 ? eA : T.e0);
+				const uint32_t kB = I.n_base + (eB < T.e1 ? eB : T.e0);
+				const uint2 nA = o.ent[kA], nB = o.ent[kB];
+				const uint32_t cA = nA.y & kMask, cB = nB.y & kMask;
+				const uint32_t oA = o.keys[kA], oB = o.keys[kB];
+				const float sA = sign_of(sc, nA.y), sB = sign_of(sc, nB.y);
+				float a0 = 0.f, a1 = 0.f, c0 = 0.f, c1 = 0.f;
+				int cur = -1;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					const uint2 pl = o.ent[jl];
+					const uint32_t prow = o.keys[jl];
+					uint32_t t = 0;
+					while (t < cnt) {
+						const int row = (int)__builtin_amdgcn_readlane(prow, t);
+						if (row != cur) {
+							if (cur >= 0) {
+								const int r = cur - w.po_begin;
+								if (vA)
+									atomicAdd(&tile[((r * nn + (int)oA) << lbt) + bl], sA * (a0 + a1));
+								if (vB)
+									atomicAdd(&tile[((r * nn + (int)oB) << lbt) + bl], sB * (c0 + c1));
+							}
+							a0 = a1 = c0 = c1 = 0.f;
+							cur = row;
+						}
+						const bool pair = t + 1 < cnt &&
+							(int)__builtin_amdgcn_readlane(prow, t + 1) == row;
+						if (pair) {
+							float va[2], vb[2], xa[2], xc[2];
+#pragma unroll
+							for (int u = 0; u < 2; u++) {
+								const uint32_t px = __builtin_amdgcn_readlane(pl.x, t + u);
+								const uint32_t py = __builtin_amdgcn_readlane(pl.y, t + u);
+								const size_t vrow = (size_t)dn * (py & kMask);
+								const size_t xrow = (size_t)nn_in * px;
+								va[u] = sign_of(Load<VT>::at(vblk, vrow + cA), py);
+								vb[u] = sign_of(Load<VT>::at(vblk, vrow + cB), py);
+								xa[u] = xb[(xrow + nA.x) * B];
+								xc[u] = xb[(xrow + nB.x) * B];
+							}
+							a0 += va[0] * xa[0];
+							a1 += va[1] * xa[1];
+							c0 += vb[0] * xc[0];
+							c1 += vb[1] * xc[1];
+							t += 2;
+						} else {
+							const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+							const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+							const size_t vrow = (size_t)dn * (py & kMask);
+							const size_t xrow = (size_t)nn_in * px;
+							a0 += sign_of(Load<VT>::at(vblk, vrow + cA), py) * xb[(xrow + nA.x) * B];
+							c0 += sign_of(Load<VT>::at(vblk, vrow + cB), py) * xb[(xrow + nB.x) * B];
+							t += 1;
+						}
+					}
+				}
+				if (cur >= 0) {
+					const int r = cur - w.po_begin;
+					if (vA)
+						atomicAdd(&tile[((r * nn + (int)oA) << lbt) + bl], sA * (a0 + a1));
+					if (vB)
+						atomicAdd(&tile[((r * nn + (int)oB) << lbt) + bl], sB * (c0 + c1));
+				}
+			}
+		} else if (I.type == 2) {
+			for (int n0 = (int)T.e0; n0 < (int)T.e1; n0 += G) {
+				const int n = n0 + slot;
+				const bool valid = n < (int)T.e1 && bvalid;
+				const uint32_t nl = n < (int)T.e1 ? (uint32_t)n : T.e0;
+				float a0 = 0.f, a1 = 0.f;
+				int cur = -1;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					const uint2 pl = o.ent[jl];
+					const uint32_t prow = o.keys[jl];
+					for (uint32_t t = 0; t < cnt; t++) {
+						const int row = (int)__builtin_amdgcn_readlane(prow, t);
+						if (row != cur) {
+							if (cur >= 0 && valid && (a0 + a1) != 0.f)
+								atomicAdd(&tile[(((cur - w.po_begin) * nn + n) << lbt) + bl], sc * (a0 + a1));
+							a0 = a1 = 0.f;
+							cur = row;
+						}
+						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+						const float e = Load<VT>::at(V, (size_t)I.v_off + (py & kMask));
+						const float xv = xb[((size_t)nn_in * px + nl) * B];
+						if (!(fabsf(e) < 1e-12f)) {
+							if (t & 1)
+								a1 += sign_of(e, py) * xv;
+							else
+								a0 += sign_of(e, py) * xv;
+						}
+					}
+				}
+				if (cur >= 0 && valid && (a0 + a1) != 0.f)
+					atomicAdd(&tile[(((cur - w.po_begin) * nn + n) << lbt) + bl], sc * (a0 + a1));
+			}
+		} else {
+			for (uint32_t k0 = T.jb; k0 < T.je; k0 += G) {
+				const uint32_t k = k0 + slot;
+				const bool valid = k < T.je && bvalid;
+				const uint32_t kk = k < T.je ? k : T.jb;
+				const uint2 en = o.ent[kk];
+				float v = Load<VT>::at(V, (size_t)I.v_off + (en.y & kMask));
+				if (!valid || fabsf(v) < 1e-12f)
+					continue;
+				v = sign_of(v, en.y) * sc;
+				const int no = (int)o.keys[kk];
+				const float *xc = xb + ((size_t)nn_in * w.po_begin + en.x) * B;
+				for (int r = 0; r < nrow; r++)
+					atomicAdd(&tile[((r * nn + no) << lbt) + bl], v * xc[(size_t)nn_in * r * B]);
+			}
+		}
+	}
+	__syncthreads();
+	float *yblk = o.y + ((size_t)w.y_off + (size_t)nn * w.po_begin) * B + b0;
+	for (int t = threadIdx.x; t < twords; t += blockDim.x) {
+		const int e = t >> lbt, c = t & (bt - 1);
+		if (b0 + c >= B)
+			continue;
+		const float v = tile[t];
+		float *dst = yblk + (size_t)e * B + c;
+		if (w.shared) {
+			if (v != 0.f)
+				atomicAdd(dst, v);
+		} else
+			*dst += v;
+	}
+}
+
+// Persistent grid over a work queue: each workgroup takes the next item
+// (heaviest first) from *next until the list is exhausted.  *next is
+// cleared by reset_queue before every launch.
+__global__ __launch_bounds__(kTileThreads) void batch_kernel(
+	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items, int nitems,
+	int *__restrict__ next)
+{
+	extern __shared__ float tile[];
+	__shared__ int s_item;
+	for (;;) {
+		if (threadIdx.x == 0)
+			s_item = atomicAdd(next, 1);
+		__syncthreads();
+		const int it = __builtin_amdgcn_readfirstlane(s_item);
+		__syncthreads();
+		if (it >= nitems)
+			break;
+		const BatchItem bi = items[it];
+		const BatchOp o = ops[bi.op];
+		const TileWork w = o.tiles[bi.tile];
+		if (o.precision == BACCHUS_HIP_BF16)
+			batch_tile<uint16_t>(o, w, bi.b0, bi.lbt, tile);
+		else
+			batch_tile<float>(o, w, bi.b0, bi.lbt, tile);
+		__syncthreads();
+	}
+}
+
+__global__ void reset_queue(int *next)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+		*next = 0;
+}
+
+__global__ void fill_float(float *p, int64_t n, float v)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+int blocks_of(int64_t n) { return (int)((n + 255) / 256); }
+
+int pow2_floor(int v)
+{
+	int p = 1;
+	while (p * 2 <= v)
+		p *= 2;
+	return p;
+}
+
+}  // namespace
+}  // namespace bacchus
+
+using namespace bacchus;
+
+struct bacchus_hip_batch {
+	std::vector<bacchus_hip_operator_t *> ops;
+	std::vector<int32_t> B;
+	bacchus_hip_batch_tuning_t tuning{};
+	std::vector<float *> d_scales;
+	std::vector<TileWork *> d_tiles;
+	std::vector<Task *> d_tasks;
+	std::vector<BatchOp> h_ops;
+	BatchOp *d_ops = nullptr;
+	BatchItem *d_items = nullptr;
+	int *d_next = nullptr;  // work-queue head
+	int64_t num_items = 0;
+	size_t lds_bytes = 0;
+	int grid = 0;
+	// internal vector buffers (autotuning, Lanczos)
+	std::vector<float *> d_x, d_y;
+	double bytes = 0, flops = 0;
+
+	void release_plan()
+	{
+		for (auto *p : d_tiles)
+			(void)hipFree(p);
+		for (auto *p : d_tasks)
+			(void)hipFree(p);
+		(void)hipFree(d_items);
+		d_tiles.clear();
+		d_tasks.clear();
+		d_items = nullptr;
+	}
+	~bacchus_hip_batch()
+	{
+		release_plan();
+		for (auto *p : d_scales)
+			(void)hipFree(p);
+		for (auto *p : d_x)
+			(void)hipFree(p);
+		for (auto *p : d_y)
+			(void)hipFree(p);
+		(void)hipFree(d_ops);
+		(void)hipFree(d_next);
+	}
+};
+
+namespace bacchus {
+namespace {
+
+int upload_ops(bacchus_hip_batch_t *bt)
+{
+	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_ops, bt->h_ops.data(),
+				    bt->h_ops.size() * sizeof(BatchOp), hipMemcpyHostToDevice));
+	return 0;
+}
+
+// Builds the tiles of every operator for the batch's tuning and the merged,
+// heaviest-first work list.
+int build_plan(bacchus_hip_batch_t *bt)
+{
+	bt->release_plan();
+	const int nops = (int)bt->ops.size();
+	const bacchus_hip_batch_tuning_t &tu = bt->tuning;
+	const int tiles_per_cu = tu.tiles_per_cu > 0 ? tu.tiles_per_cu : 64;
+	const int64_t words = tu.tile_words > 0 ? std::min<int64_t>(kBatchTileWords, tu.tile_words) : 8192;
+	const int max_vec = tu.max_tile_vectors > 0 ? pow2_floor(std::min(64, tu.max_tile_vectors)) : 64;
+	const int num_cu = bt->ops[0]->num_cu;
+	double work = 0;
+	for (int i = 0; i < nops; i++)
+		work += bt->ops[i]->mults * bt->B[i];
+	const double target = std::max(2.0e4, work / (double(num_cu) * tiles_per_cu));
+	struct Cand {
+		double cost;
+		BatchItem item;
+	};
+	std::vector<Cand> all;
+	int64_t max_words = 1;
+	for (int i = 0; i < nops; i++) {
+		const bacchus_hip_operator_t *op = bt->ops[i];
+		const int64_t nb = (int64_t)op->block_nn.size();
+		const int vec = std::min(max_vec, pow2_floor(std::max(1, bt->B[i])) *
+						    (pow2_floor(bt->B[i]) < bt->B[i] ? 2 : 1));
+		PlanParams pp;
+		pp.target = target;
+		pp.tile_elems = std::max<int64_t>(words, *std::max_element(op->block_nn.begin(),
+									    op->block_nn.end()));
+		pp.bt.resize(nb);
+		for (int64_t b = 0; b < nb; b++) {
+			int v = vec;
+			while (v > 1 && op->block_nn[b] * v > pp.tile_elems)
+				v >>= 1;
+			pp.bt[b] = v;
+		}
+		Plan plan;
+		if (int rc = make_plan(op, pp, plan))
+			return rc;
+		TileWork *dt = nullptr;
+		Task *dk = nullptr;
+		BACCHUS_HIP_CHECK(hipMalloc(&dt, std::max<size_t>(16, plan.tiles.size() * sizeof(TileWork))));
+		bt->d_tiles.push_back(dt);
+		BACCHUS_HIP_CHECK(hipMalloc(&dk, std::max<size_t>(16, plan.tasks.size() * sizeof(Task))));
+		bt->d_tasks.push_back(dk);
+		if (!plan.tiles.empty())
+			BACCHUS_HIP_CHECK(hipMemcpy(dt, plan.tiles.data(), plan.tiles.size() * sizeof(TileWork),
+						    hipMemcpyHostToDevice));
+		if (!plan.tasks.empty())
+			BACCHUS_HIP_CHECK(hipMemcpy(dk, plan.tasks.data(), plan.tasks.size() * sizeof(Task),
+						    hipMemcpyHostToDevice));
+		bt->h_ops[i].tiles = dt;
+		bt->h_ops[i].tasks = dk;
+		max_words = std::max(max_words, plan.max_tile_elems);
+		for (size_t t = 0; t < plan.tiles.size(); t++) {
+			const int v = plan.bt[t];
+			int lbt = 0;
+			while ((1 << lbt) < v)
+				lbt++;
+			for (int b0 = 0; b0 < bt->B[i]; b0 += v)
+				all.push_back(Cand{plan.cost[t], BatchItem{i, (int32_t)t, b0, lbt}});
+		}
+	}
+	std::stable_sort(all.begin(), all.end(),
+			 [](const Cand &a, const Cand &b) { return a.cost > b.cost; });
+	std::vector<BatchItem> items(all.size());
+	for (size_t k = 0; k < all.size(); k++)
+		items[k] = all[k].item;
+	if (items.size() > 0x7FFFFFFFull)
+		return BACCHUS_FAIL("more than 2^31 batch work items");
+	bt->num_items = (int64_t)items.size();
+	BACCHUS_HIP_CHECK(hipMalloc(&bt->d_items, std::max<size_t>(16, items.size() * sizeof(BatchItem))));
+	if (!items.empty())
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(BatchItem),
+					    hipMemcpyHostToDevice));
+	bt->lds_bytes = (size_t)max_words * sizeof(float);
+	const int wg_per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : 8;
+	bt->grid = (int)std::max<int64_t>(1, std::min<int64_t>(bt->num_items, (int64_t)num_cu * wg_per_cu));
+	return upload_ops(bt);
+}
+
+int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
+{
+	for (size_t i = 0; i < bt->ops.size(); i++) {
+		const int64_t n = bt->ops[i]->dimension * bt->B[i];
+		hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->h_ops[i].y, n, 0.f);
+	}
+	if (bt->num_items) {
+		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next);
+		hipLaunchKernelGGL(batch_kernel, dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
+				   bt->d_ops, bt->d_items, (int)bt->num_items, bt->d_next);
+	}
+	BACCHUS_HIP_CHECK(hipGetLastError());
+	return 0;
+}
+
+int ensure_buffers(bacchus_hip_batch_t *bt)
+{
+	for (size_t i = 0; i < bt->ops.size(); i++) {
+		const size_t n = (size_t)bt->ops[i]->dimension * bt->B[i];
+		if (!bt->d_x[i]) {
+			BACCHUS_HIP_CHECK(hipMalloc(&bt->d_x[i], n * sizeof(float)));
+			BACCHUS_HIP_CHECK(hipMemset(bt->d_x[i], 0, n * sizeof(float)));
+		}
+		if (!bt->d_y[i])
+			BACCHUS_HIP_CHECK(hipMalloc(&bt->d_y[i], n * sizeof(float)));
+	}
+	return 0;
+}
+
+// points every operator's descriptor at the internal buffers
+int use_internal(bacchus_hip_batch_t *bt)
+{
+	if (int rc = ensure_buffers(bt))
+		return rc;
+	for (size_t i = 0; i < bt->ops.size(); i++) {
+		bt->h_ops[i].x = bt->d_x[i];
+		bt->h_ops[i].y = bt->d_y[i];
+	}
+	return upload_ops(bt);
+}
+
+}  // namespace
+}  // namespace bacchus
+
+extern "C" {
+
+int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *num_vectors,
+			     int32_t num_ops, const bacchus_hip_batch_tuning_t *tuning,
+			     bacchus_hip_batch_t **out)
+{
+	if (!ops || !num_vectors || !out || num_ops <= 0)
+		return BACCHUS_FAIL("NULL argument or no operators");
+	*out = nullptr;
+	for (int i = 0; i < num_ops; i++) {
+		if (!ops[i])
+			return BACCHUS_FAIL("operator %d is NULL", i);
+		if (ops[i]->precision == BACCHUS_HIP_FP64)
+			return BACCHUS_FAIL("operator %d stores FP64 elements; the batched "
+					    "form takes FP32 or BF16 operators", i);
+		if (num_vectors[i] < 1 || num_vectors[i] > kMaxVectors)
+			return BACCHUS_FAIL("operator %d: %d vectors (1..%d allowed)", i,
+					    num_vectors[i], kMaxVectors);
+		if (ops[i]->dimension * (int64_t)num_vectors[i] > (int64_t)INT32_MAX)
+			return BACCHUS_FAIL("operator %d: D x B = %lld exceeds 2^31", i,
+					    (long long)(ops[i]->dimension * num_vectors[i]));
+		for (int64_t nn : ops[i]->block_nn)
+			if (nn > kBatchTileWords)
+				return BACCHUS_FAIL("operator %d has a block of %lld neutron states; "
+						    "the batched tile holds at most %lld", i,
+						    (long long)nn, (long long)kBatchTileWords);
+	}
+	auto *bt = new bacchus_hip_batch;
+	if (tuning)
+		bt->tuning = *tuning;
+	bt->ops.assign(ops, ops + num_ops);
+	bt->B.assign(num_vectors, num_vectors + num_ops);
+	bt->d_x.assign(num_ops, nullptr);
+	bt->d_y.assign(num_ops, nullptr);
+	bt->h_ops.resize(num_ops);
+	auto fail = [&](int rc) {
+		delete bt;
+		return rc;
+	};
+	for (int i = 0; i < num_ops; i++) {
+		const bacchus_hip_operator_t *op = ops[i];
+		const int B = num_vectors[i];
+		std::vector<float> sc((size_t)B * 4, 1.0f);
+		float *ds = nullptr;
+		if (hipMalloc(&ds, sc.size() * sizeof(float)) != hipSuccess)
+			return fail(BACCHUS_FAIL("hipMalloc of the scales failed"));
+		bt->d_scales.push_back(ds);
+		if (hipMemcpy(ds, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+			return fail(BACCHUS_FAIL("upload of the scales failed"));
+		BatchOp &o = bt->h_ops[i];
+		o = BatchOp{};
+		o.ins = op->d_tinstr;
+		o.ent = op->d_entries;
+		o.keys = op->d_keys;
+		o.V = op->d_v;
+		o.scales = ds;
+		o.B = B;
+		o.precision = op->precision;
+		const double sv = op->precision == BACCHUS_HIP_FP32 ? 4 : 2;
+		bt->bytes += sv * double(op->element_count) + 12.0 * double(op->list_entry_count) +
+			     8.0 * double(op->dimension) * B;
+		bt->flops += 2.0 * op->mults * B;
+	}
+	if (hipMalloc(&bt->d_ops, num_ops * sizeof(BatchOp)) != hipSuccess ||
+	    hipMalloc(&bt->d_next, sizeof(int)) != hipSuccess)
+		return fail(BACCHUS_FAIL("hipMalloc of the descriptors failed"));
+	if (int rc = build_plan(bt))
+		return fail(rc);
+	*out = bt;
+	return 0;
+}
+
+void bacchus_hip_batch_free(bacchus_hip_batch_t *batch) { delete batch; }
+
+int bacchus_hip_batch_set_scales(bacchus_hip_batch_t *bt, int32_t i, const double *scales)
+{
+	if (!bt || !scales)
+		return BACCHUS_FAIL("NULL argument");
+	if (i < 0 || i >= (int32_t)bt->ops.size())
+		return BACCHUS_FAIL("operator index %d out of range", i);
+	const int B = bt->B[i];
+	std::vector<float> sc((size_t)B * 4, 0.0f);
+	for (int b = 0; b < B; b++)
+		for (int f = 0; f < 3; f++)
+			sc[(size_t)b * 4 + 1 + f] = (float)scales[(size_t)b * 3 + f];
+	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_scales[i], sc.data(), sc.size() * sizeof(float),
+				    hipMemcpyHostToDevice));
+	return 0;
+}
+
+int bacchus_hip_batch_matvec(bacchus_hip_batch_t *bt, const float *const *x, float *const *y,
+			     void *stream)
+{
+	if (!bt || !x || !y)
+		return BACCHUS_FAIL("NULL argument");
+	bool changed = false;
+	for (size_t i = 0; i < bt->ops.size(); i++) {
+		if (!x[i] || !y[i])
+			return BACCHUS_FAIL("NULL vector for operator %zu", i);
+		if (bt->h_ops[i].x != x[i] || bt->h_ops[i].y != y[i]) {
+			bt->h_ops[i].x = x[i];
+			bt->h_ops[i].y = y[i];
+			changed = true;
+		}
+	}
+	if (changed) {
+		BACCHUS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+		if (int rc = upload_ops(bt))
+			return rc;
+	}
+	return launch_batch(bt, (hipStream_t)stream);
+}
+
+int bacchus_hip_batch_stats(const bacchus_hip_batch_t *bt, double *bytes, double *flops,
+			    int64_t *num_items)
+{
+	if (!bt)
+		return BACCHUS_FAIL("NULL argument");
+	if (bytes)
+		*bytes = bt->bytes;
+	if (flops)
+		*flops = bt->flops;
+	if (num_items)
+		*num_items = bt->num_items;
+	return 0;
+}
+
+int bacchus_hip_batch_autotune(bacchus_hip_batch_t *bt, int32_t reps,
+			       bacchus_hip_batch_tuning_t *best, double *best_ms)
+{
+	if (!bt)
+		return BACCHUS_FAIL("NULL argument");
+	if (reps < 1)
+		reps = 3;
+	const std::vector<BatchOp> saved = bt->h_ops;
+	if (int rc = use_internal(bt))
+		return rc;
+	hipEvent_t e0, e1;
+	BACCHUS_HIP_CHECK(hipEventCreate(&e0));
+	BACCHUS_HIP_CHECK(hipEventCreate(&e1));
+	const int tpc[] = {16, 64};
+	const int vec[] = {16, 64};
+	const int wpc[] = {4, 8};
+	const int wds[] = {4096, 8192, 16384};
+	bacchus_hip_batch_tuning_t best_t = bt->tuning;
+	double best_v = 1e300;
+	int rc = 0;
+	for (int a : tpc)
+		for (int v : vec)
+			for (int g : wpc)
+				for (int wd : wds) {
+					bacchus_hip_batch_tuning_t t{a, v, g, wd};
+					bt->tuning = t;
+					if ((rc = build_plan(bt)))
+						goto done;
+					if ((rc = launch_batch(bt, nullptr)))
+						goto done;
+					(void)hipEventRecord(e0, nullptr);
+					for (int r = 0; r < reps; r++)
+						if ((rc = launch_batch(bt, nullptr)))
+							goto done;
+					(void)hipEventRecord(e1, nullptr);
+					if (hipEventSynchronize(e1) != hipSuccess) {
+						rc = BACCHUS_FAIL("autotune run failed");
+						goto done;
+					}
+					float ms = 0;
+					(void)hipEventElapsedTime(&ms, e0, e1);
+					if (ms / reps < best_v) {
+						best_v = ms / reps;
+						best_t = t;
+					}
+				}
+	bt->tuning = best_t;
+	rc = build_plan(bt);
+done:
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	for (size_t i = 0; i < saved.size(); i++) {
+		bt->h_ops[i].x = saved[i].x;
+		bt->h_ops[i].y = saved[i].y;
+	}
+	if (!rc)
+		rc = upload_ops(bt);
+	if (rc)
+		return rc;
+	if (best)
+		*best = best_t;
+	if (best_ms)
+		*best_ms = best_v;
+	return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Batched Lanczos (src/Bacchus/lanczos/lanczos.c:74-253 per instance, with
+// full reorthogonalization instead of lanczos.c:155-169).  Vectors of an
+// operator are stored [D][B] in FP64; the Krylov basis Q[j] for j <= K is
+// kept in HBM.  Every kernel reads the iteration number k from device
+// memory, so one iteration is a fixed kernel sequence that is captured
+// once as a hipGraph and replayed.
+// ---------------------------------------------------------------------
+namespace bacchus {
+namespace {
+
+constexpr int kDotChunks = 128;
+
+struct LState {
+	int64_t D = 0;
+	int B = 0;
+	double *Q = nullptr;      // [K + 1][D][B]
+	double *w = nullptr;      // [D][B]
+	double *alpha = nullptr;  // [K][B]
+	double *beta = nullptr;   // [K][B]
+	double *h = nullptr;      // [K + 1][B]
+	double *partial = nullptr;  // [K + 1][chunks][B]
+	double *ss = nullptr;     // [B]
+	int64_t *k = nullptr;     // iteration
+};
+
+__global__ void to_float_cols(const double *__restrict__ Q, const int64_t *kp, int64_t n,
+			      float *__restrict__ xf)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		xf[i] = (float)Q[*kp * n + i];
+}
+
+__global__ void to_double_cols(const float *__restrict__ yf, int64_t n, double *__restrict__ w)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		w[i] = (double)yf[i];
+}
+
+// partial[j][c][b] = sum over rows of chunk c of U_j[r][b] * w[r][b],
+// j = 0..k (grid.y = K + 1; blocks with j > k return).  If self != 0 the
+// products are w*w (j = 0 only).  Threads: b = tid % B, row lane = tid / B.
+__global__ __launch_bounds__(256) void col_dot_partial(const double *__restrict__ U,
+						      const double *__restrict__ w, int64_t D,
+						      int B, const int64_t *kp, int self,
+						      double *__restrict__ partial)
+{
+	__shared__ double sh[256];
+	const int j = blockIdx.y;
+	if (self ? j > 0 : j > *kp)
+		return;
+	const int R = 256 / B;
+	const int tid = threadIdx.x;
+	const int b = tid % B, rl = tid / B;
+	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
+	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
+	double s = 0;
+	if (rl < R) {
+		const double *u = self ? w : U + (int64_t)j * D * B;
+		for (int64_t r = r0 + rl; r < r1; r += R) {
+			const int64_t i = r * B + b;
+			s += u[i] * w[i];
+		}
+	}
+	sh[tid] = s;
+	__syncthreads();
+	if (tid < B) {
+		double t = 0;
+		for (int q = 0; q < R; q++)
+			t += sh[tid + q * B];
+		partial[((int64_t)j * gridDim.x + blockIdx.x) * B + tid] = t;
+	}
+}
+
+// out[j][b] = sum_c partial[j][c][b] in a fixed order
+__global__ void col_dot_reduce(const double *__restrict__ partial, int chunks, int B,
+			       const int64_t *kp, int self, double *__restrict__ out)
+{
+	const int j = blockIdx.x;
+	if (self ? j > 0 : j > *kp)
+		return;
+	for (int b = threadIdx.x; b < B; b += blockDim.x) {
+		double t = 0;
+		for (int c = 0; c < chunks; c++)
+			t += partial[((int64_t)j * chunks + c) * B + b];
+		out[(int64_t)j * B + b] = t;
+	}
+}
+
+// w[r][b] -= sum_{j <= k} h[j][b] Q_j[r][b]
+__global__ void col_subtract(double *__restrict__ w, const double *__restrict__ Q,
+			     const double *__restrict__ h, int64_t n, int B, const int64_t *kp)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int b = (int)(i % B);
+	const int64_t k = *kp;
+	double v = w[i];
+	for (int64_t j = 0; j <= k; j++)
+		v -= h[j * B + b] * Q[j * n + i];
+	w[i] = v;
+}
+
+// alpha[k][b] (=|+=) h[k][b]
+__global__ void record_alpha(double *alpha, const double *h, int B, const int64_t *kp, int add)
+{
+	const int b = threadIdx.x + blockIdx.x * blockDim.x;
+	if (b >= B)
+		return;
+	const int64_t k = *kp;
+	const double v = h[k * B + b];
+	alpha[k * B + b] = add ? alpha[k * B + b] + v : v;
+}
+
+// beta[k][b] = |w_b|, Q[k+1] = w / beta (zero after a breakdown), k += 1
+__global__ void finish_iteration(double *__restrict__ Q, const double *__restrict__ w,
+				 const double *__restrict__ ss, double *__restrict__ beta,
+				 int64_t n, int B, const int64_t *kp)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int b = (int)(i % B);
+	const int64_t k = *kp;
+	const double bb = sqrt(ss[b]);
+	if (i < B)
+		beta[k * B + i] = bb;
+	Q[(k + 1) * n + i] = bb > 1e-300 ? w[i] / bb : 0.0;
+}
+
+__global__ void advance(int64_t *kp)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+		*kp += 1;
+}
+
+__global__ void set_counter(int64_t *kp, int64_t v)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+		*kp = v;
+}
+
+__device__ __forceinline__ uint64_t mix(uint64_t z)
+{
+	z += 0x9e3779b97f4a7c15ULL;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+	return z ^ (z >> 31);
+}
+
+// uniform [-1, 1) keyed by (seed, operator, flat index)
+__global__ void random_cols(double *u, int64_t n, uint64_t seed)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		u[i] = 2.0 * double(mix(seed ^ mix((uint64_t)i)) >> 11) * 0x1.0p-53 - 1.0;
+}
+
+__global__ void copy_cols(double *dst, const double *src, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[i];
+}
+
+// u[r][b] /= sqrt(ss[b])
+__global__ void normalize_cols(double *u, const double *ss, int64_t n, int B)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) {
+		const double s = sqrt(ss[i % B]);
+		u[i] = s > 0 ? u[i] / s : 0.0;
+	}
+}
+
+// out[r][b] = sum_j z[j][b] Q_j[r][b]
+__global__ void combine_cols(double *__restrict__ out, const double *__restrict__ Q,
+			     const double *__restrict__ z, int64_t n, int B, int64_t k)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int b = (int)(i % B);
+	double v = 0;
+	for (int64_t j = 0; j < k; j++)
+		v += z[j * B + b] * Q[j * n + i];
+	out[i] = v;
+}
+
+struct BatchLanczos {
+	std::vector<LState> st;
+	hipStream_t s = nullptr;
+	int64_t K = 0;
+
+	~BatchLanczos()
+	{
+		for (auto &l : st) {
+			(void)hipFree(l.Q);
+			(void)hipFree(l.w);
+			(void)hipFree(l.alpha);
+			(void)hipFree(l.beta);
+			(void)hipFree(l.h);
+			(void)hipFree(l.partial);
+			(void)hipFree(l.ss);
+			(void)hipFree(l.k);
+		}
+	}
+
+	void norm_sq(LState &l, const double *u)
+	{
+		hipLaunchKernelGGL(col_dot_partial, dim3(kDotChunks, 1), dim3(256), 0, s, nullptr, u,
+				   l.D, l.B, l.k, 1, l.partial);
+		hipLaunchKernelGGL(col_dot_reduce, dim3(1), dim3(256), 0, s, l.partial, kDotChunks, l.B,
+				   l.k, 1, l.ss);
+	}
+	void project(LState &l, int add)
+	{
+		const int64_t n = l.D * l.B;
+		hipLaunchKernelGGL(col_dot_partial, dim3(kDotChunks, (unsigned)(K + 1)), dim3(256), 0, s,
+				   l.Q, l.w, l.D, l.B, l.k, 0, l.partial);
+		hipLaunchKernelGGL(col_dot_reduce, dim3((unsigned)(K + 1)), dim3(256), 0, s, l.partial,
+				   kDotChunks, l.B, l.k, 0, l.h);
+		hipLaunchKernelGGL(col_subtract, dim3(blocks_of(n)), dim3(256), 0, s, l.w, l.Q, l.h, n,
+				   l.B, l.k);
+		hipLaunchKernelGGL(record_alpha, dim3((l.B + 255) / 256), dim3(256), 0, s, l.alpha, l.h,
+				   l.B, l.k, add);
+	}
+	// one iteration of every instance: w = H q_k, CGS2 against q_0..q_k,
+	// beta, q_{k+1}
+	int iteration(bacchus_hip_batch_t *bt)
+	{
+		for (size_t i = 0; i < st.size(); i++) {
+			LState &l = st[i];
+			const int64_t n = l.D * l.B;
+			hipLaunchKernelGGL(to_float_cols, dim3(blocks_of(n)), dim3(256), 0, s, l.Q, l.k, n,
+					   bt->d_x[i]);
+		}
+		if (int rc = launch_batch(bt, s))
+			return rc;
+		for (size_t i = 0; i < st.size(); i++) {
+			LState &l = st[i];
+			const int64_t n = l.D * l.B;
+			hipLaunchKernelGGL(to_double_cols, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_y[i], n,
+					   l.w);
+			project(l, 0);
+			project(l, 1);
+			norm_sq(l, l.w);
+			hipLaunchKernelGGL(finish_iteration, dim3(blocks_of(n)), dim3(256), 0, s, l.Q, l.w,
+					   l.ss, l.beta, n, l.B, l.k);
+			hipLaunchKernelGGL(advance, dim3(1), dim3(64), 0, s, l.k);
+		}
+		BACCHUS_HIP_CHECK(hipGetLastError());
+		return 0;
+	}
+};
+
+// lowest `count` eigenvalues of instance b's k x k tridiagonal, Krylov
+// space cut at the first breakdown
+std::vector<double> instance_eigenvalues(const std::vector<double> &alpha,
+					 const std::vector<double> &beta, int B, int b,
+					 int64_t k, int count, int64_t *used)
+{
+	std::vector<double> d, e;
+	double scale = 0;
+	for (int64_t j = 0; j < k; j++)
+		scale = std::max(scale, std::fabs(alpha[j * B + b]));
+	for (int64_t j = 0; j < k; j++) {
+		d.push_back(alpha[j * B + b]);
+		if (j + 1 < k) {
+			const double bb = beta[j * B + b];
+			if (bb <= 1e-12 * scale)
+				break;
+			e.push_back(bb);
+		}
+	}
+	if (used)
+		*used = (int64_t)d.size();
+	e.resize(d.size() > 0 ? d.size() - 1 : 0);
+	e.push_back(0.0);
+	return tridiagonal_lowest_host(d.data(), e.data(), (int64_t)d.size(), count);
+}
+
+}  // namespace
+}  // namespace bacchus
+
+extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
+					 const bacchus_hip_batch_lanczos_settings_t *set,
+					 double *eigenvalues, double *const *ritz,
+					 bacchus_hip_lanczos_result_t *result)
+{
+	if (!bt || !set || !eigenvalues)
+		return BACCHUS_FAIL("NULL argument");
+	const auto t_start = std::chrono::steady_clock::now();
+	const int nops = (int)bt->ops.size();
+	int64_t Dmin = INT64_MAX;
+	for (auto *op : bt->ops)
+		Dmin = std::min(Dmin, op->dimension);
+	const int64_t K = std::min<int64_t>(set->max_num_iterations, Dmin);
+	const int nev = set->num_eigenvalues;
+	if (K < 1 || nev < 1)
+		return BACCHUS_FAIL("need at least one iteration and one eigenvalue");
+	BatchLanczos L;
+	L.K = K;
+	if (int rc = use_internal(bt))
+		return rc;
+	BACCHUS_HIP_CHECK(hipStreamCreate(&L.s));
+	struct StreamGuard {
+		hipStream_t s;
+		~StreamGuard() { (void)hipStreamDestroy(s); }
+	} guard{L.s};
+	L.st.resize(nops);
+	for (int i = 0; i < nops; i++) {
+		LState &l = L.st[i];
+		l.D = bt->ops[i]->dimension;
+		l.B = bt->B[i];
+		const size_t n = (size_t)l.D * l.B;
+		BACCHUS_HIP_CHECK(hipMalloc(&l.Q, (size_t)(K + 1) * n * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.w, n * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.alpha, (size_t)K * l.B * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.beta, (size_t)K * l.B * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.h, (size_t)(K + 1) * l.B * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.partial, (size_t)(K + 1) * kDotChunks * l.B * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.ss, (size_t)l.B * sizeof(double)));
+		BACCHUS_HIP_CHECK(hipMalloc(&l.k, sizeof(int64_t)));
+		hipLaunchKernelGGL(set_counter, dim3(1), dim3(64), 0, L.s, l.k, (int64_t)0);
+		const double *start = set->start_vectors ? set->start_vectors[i] : nullptr;
+		if (start)
+			hipLaunchKernelGGL(copy_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, l.Q, start, (int64_t)n);
+		else
+			hipLaunchKernelGGL(random_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, l.Q, (int64_t)n,
+					   set->seed ^ (0x5bd1e995ULL * (uint64_t)(i + 1)));
+		L.norm_sq(l, l.Q);
+		hipLaunchKernelGGL(normalize_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, l.Q, l.ss,
+				   (int64_t)n, l.B);
+	}
+	BACCHUS_HIP_CHECK(hipGetLastError());
+	hipGraphExec_t exec = nullptr;
+	hipGraph_t graph = nullptr;
+	if (set->use_graph) {
+		BACCHUS_HIP_CHECK(hipStreamBeginCapture(L.s, hipStreamCaptureModeThreadLocal));
+		const int rc = L.iteration(bt);
+		hipGraph_t g = nullptr;
+		const hipError_t ec = hipStreamEndCapture(L.s, &g);
+		if (rc)
+			return rc;
+		if (ec != hipSuccess)
+			return BACCHUS_FAIL("graph capture failed: %s", hipGetErrorString(ec));
+		graph = g;
+		BACCHUS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+	}
+	struct GraphGuard {
+		hipGraph_t g;
+		hipGraphExec_t e;
+		~GraphGuard()
+		{
+			if (e)
+				(void)hipGraphExecDestroy(e);
+			if (g)
+				(void)hipGraphDestroy(g);
+		}
+	} gguard{graph, exec};
+	const int check = set->check_every > 0 ? set->check_every : 5;
+	const bool converge = set->eigenvalue_tolerance > 0;
+	std::vector<std::vector<double>> prev(nops);
+	int64_t k = 0;
+	std::vector<std::vector<double>> ha(nops), hb(nops);
+	auto read_ab = [&](int64_t kk) -> int {
+		for (int i = 0; i < nops; i++) {
+			const LState &l = L.st[i];
+			ha[i].resize((size_t)kk * l.B);
+			hb[i].resize((size_t)kk * l.B);
+			BACCHUS_HIP_CHECK(hipMemcpy(ha[i].data(), l.alpha, ha[i].size() * sizeof(double),
+						    hipMemcpyDeviceToHost));
+			BACCHUS_HIP_CHECK(hipMemcpy(hb[i].data(), l.beta, hb[i].size() * sizeof(double),
+						    hipMemcpyDeviceToHost));
+		}
+		return 0;
+	};
+	while (k < K) {
+		const int64_t steps = converge ? std::min<int64_t>(check, K - k) : K - k;
+		for (int64_t r = 0; r < steps; r++) {
+			if (exec)
+				BACCHUS_HIP_CHECK(hipGraphLaunch(exec, L.s));
+			else if (int rc = L.iteration(bt))
+				return rc;
+		}
+		k += steps;
+		BACCHUS_HIP_CHECK(hipStreamSynchronize(L.s));
+		if (!converge || k >= K)
+			continue;
+		if (int rc = read_ab(k))
+			return rc;
+		bool all = true;
+		for (int i = 0; i < nops; i++) {
+			const int B = L.st[i].B;
+			std::vector<double> cur;
+			for (int b = 0; b < B; b++) {
+				auto ev = instance_eigenvalues(ha[i], hb[i], B, b, k, nev, nullptr);
+				ev.resize(nev, NAN);
+				cur.insert(cur.end(), ev.begin(), ev.end());
+			}
+			if (prev[i].size() != cur.size())
+				all = false;
+			else
+				for (size_t q = 0; q < cur.size(); q++)
+					if (!(std::fabs(cur[q] - prev[i][q]) <=
+					      set->eigenvalue_tolerance * std::fabs(cur[q])))
+						all = false;
+			prev[i] = cur;
+		}
+		if (all)
+			break;
+	}
+	if (int rc = read_ab(k))
+		return rc;
+	size_t pos = 0;
+	for (int i = 0; i < nops; i++) {
+		LState &l = L.st[i];
+		const int B = l.B;
+		std::vector<double> z((size_t)k * B, 0.0);
+		for (int b = 0; b < B; b++) {
+			int64_t used = 0;
+			auto ev = instance_eigenvalues(ha[i], hb[i], B, b, k, nev, &used);
+			for (int q = 0; q < nev; q++)
+				eigenvalues[pos++] = q < (int)ev.size() ? ev[q] : NAN;
+			if (ritz && ritz[i] && !ev.empty()) {
+				std::vector<double> d(used), e(used > 0 ? used - 1 : 0);
+				for (int64_t j = 0; j < used; j++)
+					d[j] = ha[i][j * B + b];
+				for (int64_t j = 0; j + 1 < used; j++)
+					e[j] = hb[i][j * B + b];
+				const auto zz = tridiagonal_vector(d, e, used, ev[0]);
+				for (int64_t j = 0; j < used; j++)
+					z[(size_t)j * B + b] = zz[j];
+			}
+		}
+		if (ritz && ritz[i]) {
+			double *dz = nullptr;
+			BACCHUS_HIP_CHECK(hipMalloc(&dz, std::max<size_t>(8, z.size() * sizeof(double))));
+			BACCHUS_HIP_CHECK(hipMemcpy(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+			const int64_t n = l.D * B;
+			hipLaunchKernelGGL(combine_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, ritz[i], l.Q, dz,
+					   n, B, k);
+			const hipError_t e = hipStreamSynchronize(L.s);
+			(void)hipFree(dz);
+			if (e != hipSuccess)
+				return BACCHUS_FAIL("Ritz vectors: %s", hipGetErrorString(e));
+		}
+	}
+	BACCHUS_HIP_CHECK(hipStreamSynchronize(L.s));
+	if (result) {
+		result->iterations = k;
+		result->matvec_seconds = 0;
+		result->total_seconds =
+			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+	}
+	return 0;
+}

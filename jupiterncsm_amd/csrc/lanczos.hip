@@ -252,6 +252,8 @@ struct Lanczos {
 	}
 };
 
+}  // namespace
+
 // eigenvector of the tridiagonal T for eigenvalue lambda by inverse
 // iteration, normalized with component 0 >= 0 (diagonalization.c:55-58)
 std::vector<double> tridiagonal_vector(const std::vector<double> &d,
@@ -295,7 +297,47 @@ std::vector<double> tridiagonal_vector(const std::vector<double> &d,
 	return z;
 }
 
-}  // namespace
+
+std::vector<double> tridiagonal_lowest_host(const double *d, const double *e,
+					    int64_t k, int count)
+{
+	double lo = 1e300, hi = -1e300;
+	for (int64_t i = 0; i < k; i++) {
+		const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(e[i]) : 0.0);
+		lo = std::min(lo, d[i] - r);
+		hi = std::max(hi, d[i] + r);
+	}
+	const double span = std::max(std::fabs(lo), std::fabs(hi));
+	lo -= 1e-12 * span + 1e-300;
+	hi += 1e-12 * span + 1e-300;
+	std::vector<double> out;
+	for (int idx = 0; idx < count && idx < k; idx++) {
+		double a = lo, b = hi;
+		for (int it = 0; it < 200; it++) {
+			const double mid = 0.5 * (a + b);
+			if (mid <= a || mid >= b)
+				break;
+			int64_t cnt = 0;
+			double q = d[0] - mid;
+			if (q < 0)
+				cnt++;
+			for (int64_t i = 1; i < k; i++) {
+				if (q == 0.0)
+					q = 1e-300;
+				q = d[i] - mid - e[i - 1] * e[i - 1] / q;
+				if (q < 0)
+					cnt++;
+			}
+			if (cnt > idx)
+				b = mid;
+			else
+				a = mid;
+		}
+		out.push_back(0.5 * (a + b));
+	}
+	return out;
+}
+
 }  // namespace bacchus
 
 using namespace bacchus;

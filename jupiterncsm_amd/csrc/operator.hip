@@ -336,6 +336,149 @@ int check_list_use(ListUse &u, int64_t n_in, int64_t n_out, int64_t list)
 }  // namespace
 }  // namespace bacchus
 
+namespace bacchus {
+
+// Rows of every block are grouped into tiles of about pp.target
+// multiplications (times the tile's vector count) within the LDS budget;
+// a single overloaded group is split over instruction ranges (those tiles
+// flush with atomics).  Tiles are ordered heaviest first so the long ones
+// do not form the tail; every tile gets tasks: N instructions in entry
+// ranges over all rows, P and NP instructions as pieces of the contiguous
+// proton segment of the tile's rows times ranges of neutron entries (NP)
+// or neutron states (P).
+int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
+{
+	const std::vector<uint32_t> &rowptr = op->h_rowptr;
+	const std::vector<TInstr> &tinstr = op->h_tinstr;
+	const int64_t nb = (int64_t)op->block_np.size();
+	out = Plan();
+	std::vector<int> tile_bt;
+	for (int64_t b = 0; b < nb; b++) {
+		const int32_t first = op->blk_vi[b], last = op->blk_vi[b + 1];
+		if (first == last)
+			continue;
+		const int64_t nn = op->block_nn[b], np = op->block_np[b];
+		const int bt = pp.bt.empty() ? 1 : pp.bt[b];
+		if (nn * bt > pp.tile_elems)
+			return BACCHUS_FAIL("basis block %lld has %lld neutron states; a tile "
+					    "of %d vectors holds at most %lld", (long long)b,
+					    (long long)nn, bt, (long long)(pp.tile_elems / bt));
+		// cost of every (instruction, row) and of every row
+		const size_t nv = (size_t)(last - first);
+		std::vector<double> cost(nv * np, 0.0), row_cost(np, 0.0);
+		for (size_t k = 0; k < nv; k++) {
+			const TInstr &t = tinstr[first + k];
+			for (int64_t po = 0; po < np; po++) {
+				double c;
+				if (t.type == 3)
+					c = double(t.n_len) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
+					    t.n_len * 0.25;
+				else if (t.type == 1)
+					c = double(rowptr[t.n_base + nn] - rowptr[t.n_base]) + nn * 0.25;
+				else
+					c = double(nn) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
+					    nn * 0.25;
+				c *= bt;
+				cost[k * np + po] = c;
+				row_cost[po] += c;
+			}
+		}
+		const int64_t rows_max = std::max<int64_t>(1, pp.tile_elems / (nn * bt));
+		int64_t po0 = 0;
+		while (po0 < np) {
+			int64_t po1 = po0 + 1;
+			double c = row_cost[po0];
+			while (po1 < np && po1 - po0 < rows_max && c + row_cost[po1] <= pp.target) {
+				c += row_cost[po1];
+				po1++;
+			}
+			out.max_tile_elems = std::max(out.max_tile_elems, (po1 - po0) * nn * bt);
+			TileWork w{};
+			w.y_off = op->block_off[b];
+			w.nn_out = (int32_t)nn;
+			w.po_begin = (int32_t)po0;
+			w.po_end = (int32_t)po1;
+			const int pieces = c > 2.0 * pp.target ? (int)std::ceil(c / pp.target) : 1;
+			if (pieces == 1) {
+				w.vi_begin = first;
+				w.vi_end = last;
+				w.shared = 0;
+				out.tiles.push_back(w);
+				out.cost.push_back(c);
+				tile_bt.push_back(bt);
+			} else {
+				const double per = c / pieces;
+				int32_t begin = 0;
+				double acc = 0;
+				for (size_t k = 0; k < nv; k++) {
+					for (int64_t po = po0; po < po1; po++)
+						acc += cost[k * np + po];
+					if (acc >= per || k + 1 == nv) {
+						w.vi_begin = first + begin;
+						w.vi_end = first + (int32_t)k + 1;
+						w.shared = 1;
+						out.tiles.push_back(w);
+						out.cost.push_back(acc);
+						tile_bt.push_back(bt);
+						begin = (int32_t)k + 1;
+						acc = 0;
+					}
+				}
+			}
+			po0 = po1;
+		}
+	}
+	// heaviest first
+	std::vector<size_t> order(out.tiles.size());
+	std::iota(order.begin(), order.end(), 0);
+	std::stable_sort(order.begin(), order.end(),
+			 [&](size_t a, size_t b) { return out.cost[a] > out.cost[b]; });
+	std::vector<TileWork> tiles;
+	std::vector<double> costs;
+	std::vector<int> bts;
+	for (size_t k : order) {
+		tiles.push_back(out.tiles[k]);
+		costs.push_back(out.cost[k]);
+		bts.push_back(tile_bt[k]);
+	}
+	out.tiles.swap(tiles);
+	out.cost.swap(costs);
+	out.bt = bts;
+	for (size_t ti = 0; ti < out.tiles.size(); ti++) {
+		TileWork &t = out.tiles[ti];
+		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / bts[ti]);
+		t.t_begin = (int32_t)out.tasks.size();
+		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
+			const TInstr &I = tinstr[v];
+			if (I.type == 1) {
+				const uint32_t b = rowptr[I.n_base], e = rowptr[I.n_base + t.nn_out];
+				const uint32_t step = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(
+					kNTaskEntries, task_mults / std::max(1, t.po_end - t.po_begin)));
+				for (uint32_t k = b; k < e; k += step)
+					out.tasks.push_back(Task{v, -1, k, std::min(e, k + step), 0, 0, 0, 0});
+				continue;
+			}
+			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
+			if (je <= jb)
+				continue;
+			const uint32_t span = I.type == 3 ? I.n_len : (uint32_t)t.nn_out;
+			const uint32_t unit = I.type == 3 ? 128u : 64u;
+			const uint32_t e_step = std::min<uint32_t>((span + unit - 1) / unit * unit, 1024u);
+			const uint32_t j_step = std::max<uint32_t>(64u, (uint32_t)(task_mults / e_step));
+			for (uint32_t jj = jb; jj < je; jj += j_step)
+				for (uint32_t e = 0; e < span; e += e_step)
+					out.tasks.push_back(Task{v, 0, jj, std::min(je, jj + j_step), e,
+								 std::min(span, e + e_step), 0, 0});
+		}
+		t.t_end = (int32_t)out.tasks.size();
+		if (out.tasks.size() > 0x7FFFFFFFull)
+			return BACCHUS_FAIL("more than 2^31 tile tasks");
+	}
+	return 0;
+}
+
+}  // namespace bacchus
+
 using namespace bacchus;
 
 extern "C" {
@@ -688,186 +831,53 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		}
 	}
 
-	// --- tiles: rows of a block sized to ~target multiplications and the
-	// LDS budget; a single overloaded row is split over instruction ranges
-	// (those tiles flush with atomics) ---
-	int num_cu = 256;
+	// --- host copies for the planner; virtual instructions of a block
+	// sorted by element sub-block so neighbouring tasks share elements ---
+	op->blk_vi.assign(nb + 1, 0);
+	for (int64_t b = 0; b < nb; b++) {
+		auto &vs = per_block[b];
+		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
+			return u.v_off < w.v_off;
+		});
+		op->h_tinstr.insert(op->h_tinstr.end(), vs.begin(), vs.end());
+		op->blk_vi[b + 1] = (int32_t)op->h_tinstr.size();
+	}
+	op->h_rowptr = rowptr;
+	op->mults = mults;
 	{
 		int dev = 0;
 		hipDeviceProp_t prop;
 		if (hipGetDevice(&dev) == hipSuccess &&
 		    hipGetDeviceProperties(&prop, dev) == hipSuccess)
-			num_cu = prop.multiProcessorCount;
+			op->num_cu = prop.multiProcessorCount;
 	}
 	// ~64 tiles per CU: small tiles keep every CU busy to the end
-	double target = std::max(2.0e4, mults / (num_cu * 64.0));
+	PlanParams pp;
+	pp.target = std::max(2.0e4, mults / (op->num_cu * 64.0));
 	if (const char *tm = getenv("BACCHUS_HIP_TILE_MULTS"))
-		target = std::max(1.0e3, atof(tm));
-	std::vector<TInstr> tinstr;
-	std::vector<TileWork> tiles;
-	int64_t max_tile = 1;
-	for (int64_t b = 0; b < nb; b++) {
-		auto &vs = per_block[b];
-		if (vs.empty())
-			continue;
-		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
-			return u.v_off < w.v_off;
-		});
-		const int32_t first = (int32_t)tinstr.size();
-		tinstr.insert(tinstr.end(), vs.begin(), vs.end());
-		const int64_t nn = op->block_nn[b], np = op->block_np[b];
-		if (nn > kTileMaxElems)
-			return fail(BACCHUS_FAIL("basis block %lld has %lld neutron states; the tile "
-						 "kernel holds at most %d", (long long)b, (long long)nn,
-						 kTileMaxElems));
-		// cost of every (instruction, row) and of every row
-		std::vector<std::vector<double>> cost(vs.size());
-		std::vector<double> row_cost(np, 0.0);
-		for (size_t k = 0; k < vs.size(); k++) {
-			const TInstr &t = vs[k];
-			cost[k].assign(np, 0.0);
-			for (int64_t po = 0; po < np; po++) {
-				double c;
-				if (t.type == 3)
-					c = double(t.n_len) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
-					    t.n_len * 0.25;
-				else if (t.type == 1)
-					c = double(rowptr[t.n_base + nn] - rowptr[t.n_base]) + nn * 0.25;
-				else
-					c = double(nn) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
-					    nn * 0.25;
-				cost[k][po] = c;
-				row_cost[po] += c;
-			}
-		}
-		const int64_t rows_max = std::max<int64_t>(1, kTileMaxElems / nn);
-		int64_t po0 = 0;
-		while (po0 < np) {
-			int64_t po1 = po0 + 1;
-			double c = row_cost[po0];
-			while (po1 < np && po1 - po0 < rows_max && c + row_cost[po1] <= target) {
-				c += row_cost[po1];
-				po1++;
-			}
-			max_tile = std::max(max_tile, (po1 - po0) * nn);
-			const int pieces = c > 2.0 * target ? (int)std::ceil(c / target) : 1;
-			if (pieces == 1) {
-				TileWork w{};
-				w.y_off = op->block_off[b];
-				w.nn_out = (int32_t)nn;
-				w.po_begin = (int32_t)po0;
-				w.po_end = (int32_t)po1;
-				w.vi_begin = first;
-				w.vi_end = first + (int32_t)vs.size();
-				w.shared = 0;
-				tiles.push_back(w);
-			} else {
-				const double per = c / pieces;
-				int32_t begin = 0;
-				double acc = 0;
-				for (size_t k = 0; k < vs.size(); k++) {
-					for (int64_t po = po0; po < po1; po++)
-						acc += cost[k][po];
-					if (acc >= per || k + 1 == vs.size()) {
-						TileWork w{};
-						w.y_off = op->block_off[b];
-						w.nn_out = (int32_t)nn;
-						w.po_begin = (int32_t)po0;
-						w.po_end = (int32_t)po1;
-						w.vi_begin = first + begin;
-						w.vi_end = first + (int32_t)k + 1;
-						w.shared = 1;
-						tiles.push_back(w);
-						begin = (int32_t)k + 1;
-						acc = 0;
-					}
-				}
-			}
-			po0 = po1;
-		}
-	}
-	// heaviest tiles first, so the long ones do not form the tail
-	{
-		std::vector<double> tcost(tiles.size(), 0.0);
-		for (size_t ti = 0; ti < tiles.size(); ti++) {
-			const TileWork &t = tiles[ti];
-			for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
-				const TInstr &I = tinstr[v];
-				for (int32_t po = t.po_begin; po < t.po_end; po++) {
-					const double seg = I.type != 1 ? double(rowptr[I.p_rows + po + 1] - rowptr[I.p_rows + po]) : 0;
-					tcost[ti] += I.type == 3 ? I.n_len * seg
-						     : (I.type == 2 ? t.nn_out * seg
-								    : double(rowptr[I.n_base + t.nn_out] - rowptr[I.n_base]));
-				}
-			}
-		}
-		std::vector<size_t> order(tiles.size());
-		std::iota(order.begin(), order.end(), 0);
-		std::stable_sort(order.begin(), order.end(),
-				 [&](size_t a2, size_t b2) { return tcost[a2] > tcost[b2]; });
-		std::vector<TileWork> sorted;
-		for (size_t k : order)
-			sorted.push_back(tiles[k]);
-		tiles.swap(sorted);
-	}
-	// tasks of every tile: N instructions in entry ranges (all rows), P and
-	// NP once per row with a non-empty proton segment
-	std::vector<Task> tasks;
-	for (TileWork &t : tiles) {
-		t.t_begin = (int32_t)tasks.size();
-		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
-			const TInstr &I = tinstr[v];
-			if (I.type == 1) {
-				const uint32_t b = rowptr[I.n_base], e = rowptr[I.n_base + t.nn_out];
-				for (uint32_t k = b; k < e; k += kNTaskEntries)
-					tasks.push_back(Task{v, -1, k, std::min(e, k + kNTaskEntries), 0, 0, 0, 0});
-				continue;
-			}
-			// P and NP: the proton entries of the tile's rows form one
-			// contiguous segment; tasks cut it (and the neutron range) so
-			// each holds about kTaskMults multiplications
-			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
-			if (je <= jb)
-				continue;
-			const uint32_t span = I.type == 3 ? I.n_len : (uint32_t)t.nn_out;
-			const uint32_t unit = I.type == 3 ? 128u : 64u;
-			const uint32_t e_step = std::min<uint32_t>((span + unit - 1) / unit * unit, 1024u);
-			const uint32_t j_step = std::max<uint32_t>(64u, (uint32_t)(kTaskMults / e_step));
-			for (uint32_t jj = jb; jj < je; jj += j_step)
-				for (uint32_t e = 0; e < span; e += e_step)
-					tasks.push_back(Task{v, 0, jj, std::min(je, jj + j_step), e,
-							     std::min(span, e + e_step), 0, 0});
-		}
-		t.t_end = (int32_t)tasks.size();
-		if (tasks.size() > 0x7FFFFFFFull)
-			return fail(BACCHUS_FAIL("more than 2^31 tile tasks"));
-	}
+		pp.target = std::max(1.0e3, atof(tm));
+	pp.bt.assign(nb, 1);
+	Plan plan;
+	if (int rc = make_plan(op, pp, plan))
+		return fail(rc);
+	std::vector<TileWork> &tiles = plan.tiles;
+	std::vector<Task> &tasks = plan.tasks;
+	const std::vector<TInstr> &tinstr = op->h_tinstr;
 	op->num_tasks = (int64_t)tasks.size();
 	if (getenv("BACCHUS_HIP_VERBOSE")) {
-		// tile cost summary (multiplications per tile) for tuning
-		std::vector<double> tc;
-		for (const TileWork &t : tiles) {
-			double c = 0;
-			for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
-				const TInstr &I = tinstr[v];
-				for (int32_t po = t.po_begin; po < t.po_end; po++) {
-					const double seg = I.type != 1 ? double(rowptr[I.p_rows + po + 1] - rowptr[I.p_rows + po]) : 0;
-					c += I.type == 3 ? I.n_len * seg : (I.type == 2 ? t.nn_out * seg : double(rowptr[I.n_base + t.nn_out] - rowptr[I.n_base]));
-				}
-			}
-			tc.push_back(c);
-		}
+		std::vector<double> tc = plan.cost;
 		std::sort(tc.begin(), tc.end());
 		double sum = 0;
 		for (double c : tc)
 			sum += c;
-		fprintf(stderr, "tiles %zu tasks %zu target %.3g sum %.4g mean %.3g p50 %.3g p90 %.3g p99 %.3g max %.3g\n",
-			tc.size(), tasks.size(), target, sum, sum / tc.size(), tc[tc.size() / 2],
-			tc[tc.size() * 9 / 10], tc[tc.size() * 99 / 100], tc.back());
+		if (!tc.empty())
+			fprintf(stderr, "tiles %zu tasks %zu target %.3g sum %.4g mean %.3g p50 %.3g p90 %.3g p99 %.3g max %.3g\n",
+				tc.size(), tasks.size(), pp.target, sum, sum / tc.size(), tc[tc.size() / 2],
+				tc[tc.size() * 9 / 10], tc[tc.size() * 99 / 100], tc.back());
 	}
 	op->num_tiles = (int64_t)tiles.size();
 	op->num_tinstr = (int64_t)tinstr.size();
-	op->tile_lds_bytes = (size_t)max_tile * sizeof(double);
+	op->tile_lds_bytes = (size_t)plan.max_tile_elems * sizeof(double);
 
 	// --- statistics: compulsory bytes of referenced arrays ---
 	std::vector<char> vref(nv, 0);
