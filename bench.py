@@ -9,7 +9,18 @@ against FP64 reference eigenvalues (the FP64 GPU path run to 300 Lanczos
 iterations with full reorthogonalization) with the fast path: FP32-stored
 matrix elements with FP32 products and accumulation, FP64 Lanczos
 recurrence, full FP64 reorthogonalization and the FP64 tridiagonal solve on
-the GPU.  The plain FP64 GPU path is timed to the same criterion beside it.
+the GPU, started from the learned start vector (an MLP over a stored basis,
+trained beforehand on batched solves of the family H(c1, c2) = H1 + c1 V2 +
+c2 V3, untimed).  The plain FP64 GPU path (random start) is timed to the
+same criterion beside it.
+
+Other modes (one JSON line each):
+  --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
+              8 in one batched Lanczos (persistent batched matvec, captured
+              graph) against 64 sequential plain FP64 solves;
+  --learned   the learned start of the 3+3 Nmax 6 ground state: 512 batched
+              training solves, 64 held-out points, iterations and time to
+              1e-6 of the lowest eigenvalue against a random start.
 
 The line also carries the roofline of the dominant kernel (the FP64 matvec,
 HBM-bound: algorithmic bytes = 8 sum|V| + 12 sum|L| + 16 D per launch over
@@ -39,7 +50,15 @@ def parse():
     ap.add_argument("--eigenvalues", type=int, default=5)
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--matvecs", type=int, default=20, help="timed matvecs for the roofline")
-    ap.add_argument("--cpu-matvecs", type=int, default=3)
+    ap.add_argument("--cpu-matvecs", type=int, default=60)
+    ap.add_argument("--train-points", type=int, default=128,
+                    help="family solves the learned start is trained on")
+    ap.add_argument("--train-iterations", type=int, default=80)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="fast path from a random start instead of the learned one")
+    ap.add_argument("--batched", action="store_true", help="batched 64-Hamiltonian sweep")
+    ap.add_argument("--learned", action="store_true", help="learned-start measurement")
+    ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only run this many FP64 matvecs (for profilers) and exit")
@@ -88,6 +107,10 @@ def main():
         torch.cuda.synchronize()
         log(f"copied {args.copy_calibration} MiB x 3")
         return
+    if args.batched:
+        return batched_sweep(args)
+    if args.learned:
+        return learned_start(args)
     t0 = time.time()
     inputs = build(SYSTEMS[args.workload])
     log(f"inputs {args.workload}: {inputs.summary()} in {time.time() - t0:.1f}s")
@@ -118,6 +141,15 @@ def main():
         return
 
     op32 = bacchus.Operator.from_inputs(inputs, bacchus.FP32)
+    for _ in range(3):
+        op32.matvec(x, y, stream=stream)
+    e0.record(stream)
+    for _ in range(n_mv):
+        op32.matvec(x, y, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    mv32_ms = e0.elapsed_time(e1) / n_mv
+    log(f"fp32 matvec {mv32_ms:.3f} ms")
 
     # --- FP64 reference eigenvalues (GPU FP64 path, full length) --------
     t0 = time.time()
@@ -128,15 +160,33 @@ def main():
     targets = ref.eigenvalues
     log(f"reference eigenvalues ({ref.iterations} its, {time.time() - t0:.2f}s): {targets}")
 
-    def solve(op, seed):
+    # --- learned start: trained on batched solves of the family (untimed) -
+    model = None
+    if not args.no_learned:
+        from jupiterncsm_amd import learned as L
+        t0 = time.time()
+        pts = L.family_points(args.train_points, 1)
+        sol = L.solve_family(op32, pts, iterations=args.train_iterations,
+                             num_eigenvalues=args.eigenvalues, target="band",
+                             band=args.eigenvalues)
+        model = L.fit(sol, basis_size=64, steps=3000, target="band")
+        log(f"learned start: {args.train_points} batched family solves "
+            f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, "
+            f"total {time.time() - t0:.1f}s")
+
+    def solve(op, seed, learned=False):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        r = bacchus.diagonalize(op, args.iterations, num_eigenvalues=args.eigenvalues,
-                                start=bacchus.START_RANDOM, seed=seed,
-                                reorthogonalization=bacchus.REORTH_FULL,
-                                convergence=bacchus.CONVERGE_TARGETS,
-                                tolerance=args.tolerance, target_eigenvalues=targets,
-                                check_every=1)
+        kw = dict(num_eigenvalues=args.eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
+                  convergence=bacchus.CONVERGE_TARGETS, tolerance=args.tolerance,
+                  target_eigenvalues=targets, check_every=1)
+        sv = model.start_vector(1.0, 1.0, mix=0.01) if learned else None
+        if sv is not None:
+            r = bacchus.diagonalize(op, args.iterations, start=bacchus.START_GIVEN,
+                                    start_vector=sv, **kw)
+        else:
+            r = bacchus.diagonalize(op, args.iterations, start=bacchus.START_RANDOM,
+                                    seed=seed, **kw)
         torch.cuda.synchronize()
         return time.perf_counter() - t, r
 
@@ -146,11 +196,12 @@ def main():
     log(f"fp64 path: {t64:.3f}s, {r64.iterations} its, errors "
         f"{np.abs(r64.eigenvalues - targets) / np.abs(targets)}")
 
+    use_learned = model is not None
     for _ in range(args.warmup):
-        solve(op32, 2)
+        solve(op32, 2, use_learned)
     times, last = [], None
     for _ in range(args.steps):
-        dt, last = solve(op32, 2)
+        dt, last = solve(op32, 2, use_learned)
         times.append(dt)
     t_fast = float(np.mean(times))
     errs = np.abs(last.eigenvalues - targets) / np.abs(targets)
@@ -193,7 +244,10 @@ def main():
         "config": {"workload": f"{args.workload}: 3p+3n Nmax6 M=0 +parity, 2+3-body, "
                                f"D={D}", "eigenvalues": args.eigenvalues,
                    "tolerance": args.tolerance, "max_iterations": args.iterations,
-                   "fast_path": "fp32 elements + fp64 lanczos, full reorth, random start"},
+                   "fast_path": "fp32 elements + fp64 lanczos, full reorth, " +
+                                ("learned start (MLP over 64 stored band vectors, trained "
+                                 f"on {args.train_points} batched family solves)"
+                                 if use_learned else "random start")},
         "fp64_gpu_seconds": round(t64, 4),
         "fp64_gpu_iterations": r64.iterations,
         "fast_iterations": last.iterations,
@@ -204,10 +258,174 @@ def main():
         "relative_errors_fp64": [float(v) for v in
                                  np.abs(r64.eigenvalues - targets) / np.abs(targets)],
         "matvec_fp64_ms": round(mv_ms, 4),
+        "matvec_fp32_ms": round(mv32_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(args.workload)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def batched_sweep(args):
+    """64 Hamiltonians H(c1, c2) spread over the 2+2 systems at Nmax 4, 6,
+    8: one batched Lanczos (all matvecs of an iteration in one persistent
+    launch, each iteration a replayed graph) against sequential plain FP64
+    single-vector solves, same Krylov length, lowest 3 eigenvalues."""
+    import numpy as np
+    import torch
+
+    from jupiterncsm_amd import bacchus
+    from jupiterncsm_amd import learned as L
+    from jupiterncsm_amd.inputs import SYSTEMS, build
+
+    names = ["he4_nmax4", "he4_nmax6", "he4_nmax8"]
+    inps = [build(SYSTEMS[n]) for n in names]
+    pts = L.family_points(args.instances, 4)
+    split = np.array_split(np.arange(args.instances), len(names))
+    counts = [len(s) for s in split]
+    K, nev = 120, 3
+    ops32 = [bacchus.Operator.from_inputs(i, bacchus.FP32) for i in inps]
+    bt = bacchus.Batch(ops32, counts)
+    for k, s in enumerate(split):
+        bt.set_scales(k, L.scales_of(pts[s]))
+    tune = bt.autotune(3)
+    st = bt.stats()
+    log(f"batch: {counts} instances of D={[i.dimension for i in inps]}, {st['work_items']} "
+        f"work items, autotuned {tune}")
+    # batched matvec rate
+    xs = [torch.randn(i.dimension, b, dtype=torch.float32, device="cuda") for i, b in
+          zip(inps, counts)]
+    ys = [torch.empty_like(x) for x in xs]
+    for _ in range(3):
+        bt.matvec(xs, ys)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        bt.matvec(xs, ys)
+    e1.record()
+    torch.cuda.synchronize()
+    bmv = e0.elapsed_time(e1) / 20
+    for _ in range(args.warmup):
+        bt.lanczos(K, num_eigenvalues=nev, use_graph=True, seed=5)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ev, its, _ = bt.lanczos(K, num_eigenvalues=nev, use_graph=True, seed=5)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    t_batch = float(np.mean(times))
+    ev_e, _, t_eager = bt.lanczos(K, num_eigenvalues=nev, use_graph=False, seed=5)
+    log(f"batched: {t_batch:.3f}s for {args.instances} solves ({its} its each), eager "
+        f"{t_eager:.3f}s")
+    # sequential plain FP64 solves
+    ops64 = [bacchus.Operator.from_inputs(i, bacchus.FP64) for i in inps]
+    errs = []
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq = []
+    for k, s in enumerate(split):
+        for b, p in enumerate(pts[s]):
+            ops64[k].set_scales(1.0, p[0], p[1])
+            r = bacchus.diagonalize(ops64[k], K, num_eigenvalues=nev,
+                                    start=bacchus.START_RANDOM, seed=5,
+                                    reorthogonalization=bacchus.REORTH_FULL,
+                                    convergence=bacchus.CONVERGE_NONE)
+            seq.append((k, b, r.eigenvalues))
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    for k, b, e in seq:
+        errs.append(float(np.max(np.abs(ev[k][b] - e) / np.abs(e))))
+    log(f"sequential fp64: {t_seq:.3f}s; max rel diff batched vs fp64 {max(errs):.2e}")
+    line = {
+        "metric": "batched sweep: 64 H(c1,c2) lowest 3 eigenvalues, solves/s",
+        "value": round(args.instances / t_batch, 2), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 2),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 Hamiltonians H1 + c1 V2 + c2 V3, c ~ U[0.8,1.2], over "
+                               "2+2 Nmax 4/6/8", "instances": counts,
+                   "dimensions": [i.dimension for i in inps], "krylov": K, "eigenvalues": nev},
+        "sequential_fp64_seconds": round(t_seq, 3),
+        "sequential_solves_per_s": round(args.instances / t_seq, 2),
+        "speedup_vs_sequential": round(t_seq / t_batch, 2),
+        "eager_seconds": round(t_eager, 3),
+        "batched_matvec_ms": round(bmv, 4),
+        "batched_matvec_gflops": round(st["flops"] / bmv / 1e6, 1),
+        "batched_matvec_algorithmic_gbs": round(st["algorithmic_bytes"] / bmv / 1e6, 1),
+        "autotuned": tune,
+        "max_rel_diff_vs_fp64_solves": max(errs),
+        "mfma": "not used: the batched products are sparse gathers (see design notes)",
+    }
+    print(json.dumps(line), flush=True)
+
+
+def learned_start(args):
+    """Learned start of the 3+3 Nmax 6 ground state: trained on 512 batched
+    family solves, tested on 64 held-out (c1, c2) points."""
+    import numpy as np
+    import torch
+
+    from jupiterncsm_amd import bacchus
+    from jupiterncsm_amd import learned as L
+    from jupiterncsm_amd.inputs import SYSTEMS, build
+
+    inputs = build(SYSTEMS["li6_nmax6_3b"])
+    op32 = bacchus.Operator.from_inputs(inputs, bacchus.FP32)
+    train = L.family_points(512, 1)
+    test = L.family_points(64, 2)
+    t0 = time.time()
+    ref = L.solve_family(op32, test, iterations=300, num_eigenvalues=1, keep_targets=False)
+    log(f"held-out references: {ref.seconds:.1f}s")
+    sol = L.solve_family(op32, train, iterations=60, num_eigenvalues=1, target="ground")
+    log(f"training solves: {sol.seconds:.1f}s")
+    model = L.fit(sol, basis_size=64, steps=6000, target="ground")
+    log(f"MLP fit: loss {model.train_loss:.2e}, total {time.time() - t0:.1f}s")
+    rows = []
+    fallbacks = 0
+    for i, c in enumerate(test):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sv = model.start_vector(*c, mix=0.01)
+        fallbacks += sv is None
+        rl = L.solve_from(op32, c, sv, num_eigenvalues=1, targets=ref.eigenvalues[i][:1])
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - t
+        t = time.perf_counter()
+        rr = L.solve_from(op32, c, None, num_eigenvalues=1, targets=ref.eigenvalues[i][:1])
+        torch.cuda.synchronize()
+        tr = time.perf_counter() - t
+        rows.append((rl.iterations, tl, rr.iterations, tr))
+    a = np.array(rows, dtype=float)
+    log(f"learned: {a[:, 0].mean():.1f} its {a[:, 1].mean() * 1e3:.1f} ms; random: "
+        f"{a[:, 2].mean():.1f} its {a[:, 3].mean() * 1e3:.1f} ms")
+    for i, r in enumerate(rows):
+        log(f"  c=({test[i][0]:.4f},{test[i][1]:.4f}) learned {r[0]} its {r[1] * 1e3:.1f} ms, "
+            f"random {r[2]} its {r[3] * 1e3:.1f} ms")
+    line = {
+        "metric": "learned start: Lanczos iterations to 1e-6 of the lowest eigenvalue",
+        "value": round(float(a[:, 0].mean()), 2), "unit": "iterations", "n_gpus": 1,
+        "steps": len(test), "warmup": 0, "ms_per_step": round(float(a[:, 1].mean()) * 1e3, 2),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "li6_nmax6_3b family H1 + c1 V2 + c2 V3, c ~ U[0.8,1.2]",
+                   "train_points": 512, "test_points": len(test), "basis": 64,
+                   "mlp": "2-256-256-256-64 GELU, BF16 weights, FP32 accumulate"},
+        "random_start_iterations": round(float(a[:, 2].mean()), 2),
+        "learned_ms": round(float(a[:, 1].mean()) * 1e3, 2),
+        "random_ms": round(float(a[:, 3].mean()) * 1e3, 2),
+        "iterations_ratio": round(float(a[:, 2].mean() / a[:, 0].mean()), 2),
+        "time_ratio": round(float(a[:, 3].mean() / a[:, 1].mean()), 2),
+        "max_learned_iterations": int(a[:, 0].max()),
+        "median_learned_iterations": float(np.median(a[:, 0])),
+        "median_random_iterations": float(np.median(a[:, 2])),
+        "points_not_converged_learned": int((a[:, 0] >= 300).sum()),
+        "points_not_converged_random": int((a[:, 2] >= 300).sum()),
+        "fallbacks": int(fallbacks),
+        "train_loss": model.train_loss,
+        "training_seconds": round(sol.seconds, 1),
     }
     print(json.dumps(line), flush=True)
 

@@ -213,7 +213,7 @@ typedef struct bacchus_hip_batch_lanczos_settings {
 					num_eigenvalues moved less than this
 					(relative) between checks */
 	int32_t check_every;
-	int32_t pad;
+	int32_t num_ritz_vectors;  /* Ritz vectors per instance (0 = 1)      */
 	uint64_t seed;             /* random starts                          */
 	const double *const *start_vectors; /* NULL or per operator NULL/device
 					       FP64 [D_i][B_i]             */
@@ -221,12 +221,25 @@ typedef struct bacchus_hip_batch_lanczos_settings {
 
 /* eigenvalues: host, sum_i B_i * num_eigenvalues, operator-major then
  * vector then ascending eigenvalue; ritz_vectors: NULL or per operator
- * NULL/device FP64 [D_i][B_i] receiving each instance's lowest Ritz
- * vector (unit norm). */
+ * NULL/device FP64 [num_ritz_vectors][D_i][B_i] receiving each instance's
+ * lowest Ritz vectors (unit norm, ascending eigenvalue). */
 int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *batch,
 			      const bacchus_hip_batch_lanczos_settings_t *settings,
 			      double *eigenvalues, double *const *ritz_vectors,
 			      bacchus_hip_lanczos_result_t *result);
+
+/* Stored-basis helpers of the learned start (no counterpart in the
+ * reference).  U holds n FP32 vectors of dimension D ([n][D]).
+ *   gram:      G[i][j] = U_i . U_j                  (host, n x n, FP64)
+ *   project:   out[i][b] = U_i . V[:, b]            (host, n x B, FP64;
+ *              V device FP64 [D][B])
+ *   combine:   out = sum_i c_i U_i / |sum_i c_i U_i| (c device FP32, n;
+ *              out device FP64, D) */
+int bacchus_hip_basis_gram(const float *U, int64_t n, int64_t D, double *G);
+int bacchus_hip_basis_project(const float *U, int64_t n, int64_t D,
+			      const double *V, int64_t B, double *out);
+int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D,
+			      const float *c, double *out, void *stream);
 
 /* Learned Lanczos start (no counterpart in the reference): an MLP
  * in -> hidden (x depth, GELU) -> out with BF16 weights on MFMA, FP32

@@ -47,7 +47,8 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_mlp_forward", "bacchus_hip_mlp_gradients", "bacchus_hip_mlp_train_step",
     "bacchus_hip_batch_create", "bacchus_hip_batch_free", "bacchus_hip_batch_set_scales",
     "bacchus_hip_batch_matvec", "bacchus_hip_batch_autotune", "bacchus_hip_batch_stats",
-    "bacchus_hip_batch_lanczos",
+    "bacchus_hip_batch_lanczos", "bacchus_hip_basis_gram", "bacchus_hip_basis_project",
+    "bacchus_hip_basis_combine",
 )
 
 
@@ -86,7 +87,7 @@ class _BatchTuning(ctypes.Structure):
 class _BatchLanczosSettings(ctypes.Structure):
     _fields_ = [("max_num_iterations", ctypes.c_int64), ("num_eigenvalues", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("eigenvalue_tolerance", ctypes.c_double),
-                ("check_every", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("check_every", ctypes.c_int32), ("num_ritz_vectors", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("start_vectors", ctypes.c_void_p)]
 
 
@@ -159,6 +160,12 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_batch_lanczos.restype = c_int
     lib.bacchus_hip_batch_lanczos.argtypes = [vp, ctypes.POINTER(_BatchLanczosSettings), vp, vp,
                                               ctypes.POINTER(_LanczosResult)]
+    lib.bacchus_hip_basis_gram.restype = c_int
+    lib.bacchus_hip_basis_gram.argtypes = [vp, i64, i64, vp]
+    lib.bacchus_hip_basis_project.restype = c_int
+    lib.bacchus_hip_basis_project.argtypes = [vp, i64, i64, vp, i64, vp]
+    lib.bacchus_hip_basis_combine.restype = c_int
+    lib.bacchus_hip_basis_combine.argtypes = [vp, i64, i64, vp, vp, vp]
     _LIB = lib
     return lib
 
@@ -398,18 +405,21 @@ class Batch:
         return {"algorithmic_bytes": b.value, "flops": f.value, "work_items": w.value}
 
     def lanczos(self, max_num_iterations, num_eigenvalues=3, tolerance=0.0, check_every=5,
-                use_graph=True, seed=1, start_vectors=None, ritz_vectors=None):
+                use_graph=True, seed=1, start_vectors=None, ritz_vectors=None, num_ritz=1):
         """Independent Lanczos recurrences for every (operator, vector).
         Returns (eigenvalues, iterations, seconds); eigenvalues is a list of
-        (B_i, num_eigenvalues) arrays.  start_vectors / ritz_vectors: lists
-        of float64 device tensors (D_i, B_i) or None."""
+        (B_i, num_eigenvalues) arrays.  start_vectors: list of float64
+        device tensors (D_i, B_i) or None; ritz_vectors: list of float64
+        device tensors (num_ritz, D_i, B_i) or None receiving the lowest
+        num_ritz Ritz vectors of every instance."""
         n = len(self.operators)
         sv = None
         if start_vectors is not None:
             sv = (ctypes.c_void_p * n)(*[t.data_ptr() if t is not None else None
                                          for t in start_vectors])
         st = _BatchLanczosSettings(int(max_num_iterations), int(num_eigenvalues),
-                                   int(bool(use_graph)), float(tolerance), int(check_every), 0,
+                                   int(bool(use_graph)), float(tolerance), int(check_every),
+                                   int(num_ritz),
                                    int(seed), ctypes.cast(sv, ctypes.c_void_p) if sv else None)
         total = sum(self.num_vectors) * num_eigenvalues
         ev = np.zeros(total)
@@ -443,6 +453,49 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _tensor(t, dtype, what):
+    import torch
+    if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+        raise BacchusError(f"{what} must be a contiguous {dtype} device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def basis_gram(U):
+    """G = U U^T (FP64, host) for a float32 device tensor U of shape (n, D)."""
+    import torch
+    n, D = U.shape
+    G = np.zeros((n, n))
+    _check(library().bacchus_hip_basis_gram(_tensor(U, torch.float32, "U"), n, D, _ptr(G)))
+    return G
+
+
+def basis_project(U, V):
+    """U V (FP64, host (n, B)) for U float32 (n, D) and V float64 (D, B)."""
+    import torch
+    n, D = U.shape
+    V2 = V.reshape(D, -1)
+    out = np.zeros((n, V2.shape[1]))
+    _check(library().bacchus_hip_basis_project(_tensor(U, torch.float32, "U"), n, D,
+                                               _tensor(V2, torch.float64, "V"), V2.shape[1],
+                                               _ptr(out)))
+    return out
+
+
+def basis_combine(U, c, out=None, stream=None):
+    """out = normalize(sum_i c_i U_i) on the device; c float32 (n,)."""
+    import torch
+    n, D = U.shape
+    if out is None:
+        out = torch.empty(D, dtype=torch.float64, device=U.device)
+    if c.numel() != n:
+        raise BacchusError(f"need {n} coefficients")
+    _check(library().bacchus_hip_basis_combine(_tensor(U, torch.float32, "U"), n, D,
+                                               _tensor(c, torch.float32, "c"),
+                                               _tensor(out, torch.float64, "out"),
+                                               _stream_handle(stream)))
+    return out
 
 
 class MLP:

@@ -1087,34 +1087,39 @@ extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
 	}
 	if (int rc = read_ab(k))
 		return rc;
+	const int nritz = std::max(1, set->num_ritz_vectors);
 	size_t pos = 0;
 	for (int i = 0; i < nops; i++) {
 		LState &l = L.st[i];
 		const int B = l.B;
-		std::vector<double> z((size_t)k * B, 0.0);
+		const int64_t n = l.D * B;
+		// z[r][j][b]: Krylov coefficients of Ritz vector r of instance b
+		std::vector<double> z((size_t)nritz * k * B, 0.0);
 		for (int b = 0; b < B; b++) {
 			int64_t used = 0;
-			auto ev = instance_eigenvalues(ha[i], hb[i], B, b, k, nev, &used);
+			auto ev = instance_eigenvalues(ha[i], hb[i], B, b, k, std::max(nev, nritz), &used);
 			for (int q = 0; q < nev; q++)
 				eigenvalues[pos++] = q < (int)ev.size() ? ev[q] : NAN;
-			if (ritz && ritz[i] && !ev.empty()) {
-				std::vector<double> d(used), e(used > 0 ? used - 1 : 0);
+			if (!ritz || !ritz[i])
+				continue;
+			std::vector<double> d(used), e(used > 0 ? used - 1 : 0);
+			for (int64_t j = 0; j < used; j++)
+				d[j] = ha[i][j * B + b];
+			for (int64_t j = 0; j + 1 < used; j++)
+				e[j] = hb[i][j * B + b];
+			for (int r = 0; r < nritz && r < (int)ev.size(); r++) {
+				const auto zz = tridiagonal_vector(d, e, used, ev[r]);
 				for (int64_t j = 0; j < used; j++)
-					d[j] = ha[i][j * B + b];
-				for (int64_t j = 0; j + 1 < used; j++)
-					e[j] = hb[i][j * B + b];
-				const auto zz = tridiagonal_vector(d, e, used, ev[0]);
-				for (int64_t j = 0; j < used; j++)
-					z[(size_t)j * B + b] = zz[j];
+					z[((size_t)r * k + j) * B + b] = zz[j];
 			}
 		}
 		if (ritz && ritz[i]) {
 			double *dz = nullptr;
 			BACCHUS_HIP_CHECK(hipMalloc(&dz, std::max<size_t>(8, z.size() * sizeof(double))));
 			BACCHUS_HIP_CHECK(hipMemcpy(dz, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
-			const int64_t n = l.D * B;
-			hipLaunchKernelGGL(combine_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, ritz[i], l.Q, dz,
-					   n, B, k);
+			for (int r = 0; r < nritz; r++)
+				hipLaunchKernelGGL(combine_cols, dim3(blocks_of(n)), dim3(256), 0, L.s,
+						   ritz[i] + (size_t)r * n, l.Q, dz + (size_t)r * k * B, n, B, k);
 			const hipError_t e = hipStreamSynchronize(L.s);
 			(void)hipFree(dz);
 			if (e != hipSuccess)

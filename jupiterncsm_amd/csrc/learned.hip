@@ -1,0 +1,175 @@
+// learned.hip - the stored-basis side of the learned Lanczos start: Gram
+// matrix and projections that turn solved eigenvectors into training
+// targets, and the combination that turns the network's coefficients into
+// a start vector on the device.  FP32 storage, FP64 accumulation.
+#include "bacchus_internal.h"
+
+#include <vector>
+
+namespace bacchus {
+namespace {
+
+constexpr int kChunks = 64;
+
+__device__ double block_reduce(double v, double *sh)
+{
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o, 64);
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	if (lane == 0)
+		sh[wid] = v;
+	__syncthreads();
+	double t = 0;
+	if (threadIdx.x == 0)
+		for (int w = 0; w < (int)(blockDim.x >> 6); w++)
+			t += sh[w];
+	__syncthreads();
+	return t;
+}
+
+// partial[(i * n + j) * chunks + c] = U_i . U_j over chunk c (j <= i)
+__global__ __launch_bounds__(256) void gram_partial(const float *__restrict__ U, int n, int64_t D,
+						   double *__restrict__ partial)
+{
+	__shared__ double sh[4];
+	const int i = blockIdx.y / n, j = blockIdx.y % n;
+	if (j > i)
+		return;
+	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
+	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
+	const float *a = U + (int64_t)i * D, *b = U + (int64_t)j * D;
+	double s = 0;
+	for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+		s += (double)a[r] * (double)b[r];
+	s = block_reduce(s, sh);
+	if (threadIdx.x == 0)
+		partial[((int64_t)i * n + j) * gridDim.x + blockIdx.x] = s;
+}
+
+// partial[(i * chunks + c) * B + b] = U_i . V[:, b] over chunk c
+__global__ __launch_bounds__(256) void project_partial(const float *__restrict__ U, int64_t D,
+						      const double *__restrict__ V, int B,
+						      double *__restrict__ partial)
+{
+	__shared__ double sh[256];
+	const int i = blockIdx.y;
+	const int R = 256 / B;
+	const int b = threadIdx.x % B, rl = threadIdx.x / B;
+	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
+	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
+	const float *u = U + (int64_t)i * D;
+	double s = 0;
+	if (rl < R)
+		for (int64_t r = r0 + rl; r < r1; r += R)
+			s += (double)u[r] * V[r * B + b];
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	if ((int)threadIdx.x < B) {
+		double t = 0;
+		for (int q = 0; q < R; q++)
+			t += sh[threadIdx.x + q * B];
+		partial[((int64_t)i * gridDim.x + blockIdx.x) * B + threadIdx.x] = t;
+	}
+}
+
+// out[r] = sum_i c_i U_i[r] (FP64 accumulation)
+__global__ void combine_basis(const float *__restrict__ U, int n, int64_t D,
+			      const float *__restrict__ c, double *__restrict__ out)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= D)
+		return;
+	double v = 0;
+	for (int i = 0; i < n; i++)
+		v += (double)c[i] * (double)U[(int64_t)i * D + r];
+	out[r] = v;
+}
+
+// one block: ss = |out|^2 in a fixed order, then out /= sqrt(ss)
+__global__ __launch_bounds__(256) void normalize_vector(double *out, int64_t D)
+{
+	__shared__ double sh[4];
+	__shared__ double total;
+	double s = 0;
+	for (int64_t r = threadIdx.x; r < D; r += blockDim.x)
+		s += out[r] * out[r];
+	s = block_reduce(s, sh);
+	if (threadIdx.x == 0)
+		total = s;
+	__syncthreads();
+	const double inv = total > 0 ? 1.0 / sqrt(total) : 0.0;
+	for (int64_t r = threadIdx.x; r < D; r += blockDim.x)
+		out[r] *= inv;
+}
+
+}  // namespace
+}  // namespace bacchus
+
+using namespace bacchus;
+
+extern "C" {
+
+int bacchus_hip_basis_gram(const float *U, int64_t n, int64_t D, double *G)
+{
+	if (!U || !G || n <= 0 || D <= 0 || n > 255)
+		return BACCHUS_FAIL("bad arguments (n = %lld, D = %lld)", (long long)n, (long long)D);
+	double *dp = nullptr;
+	const size_t np = (size_t)n * n * kChunks;
+	BACCHUS_HIP_CHECK(hipMalloc(&dp, np * sizeof(double)));
+	hipLaunchKernelGGL(gram_partial, dim3(kChunks, (unsigned)(n * n)), dim3(256), 0, nullptr, U,
+			   (int)n, D, dp);
+	std::vector<double> h(np);
+	const hipError_t e = hipMemcpy(h.data(), dp, np * sizeof(double), hipMemcpyDeviceToHost);
+	(void)hipFree(dp);
+	if (e != hipSuccess)
+		return BACCHUS_FAIL("gram: %s", hipGetErrorString(e));
+	for (int64_t i = 0; i < n; i++)
+		for (int64_t j = 0; j <= i; j++) {
+			double s = 0;
+			for (int c = 0; c < kChunks; c++)
+				s += h[(i * n + j) * kChunks + c];
+			G[i * n + j] = G[j * n + i] = s;
+		}
+	return 0;
+}
+
+int bacchus_hip_basis_project(const float *U, int64_t n, int64_t D, const double *V, int64_t B,
+			      double *out)
+{
+	if (!U || !V || !out || n <= 0 || D <= 0 || B <= 0 || B > 256)
+		return BACCHUS_FAIL("bad arguments (n = %lld, D = %lld, B = %lld)", (long long)n,
+				    (long long)D, (long long)B);
+	double *dp = nullptr;
+	const size_t np = (size_t)n * kChunks * B;
+	BACCHUS_HIP_CHECK(hipMalloc(&dp, np * sizeof(double)));
+	hipLaunchKernelGGL(project_partial, dim3(kChunks, (unsigned)n), dim3(256), 0, nullptr, U, D, V,
+			   (int)B, dp);
+	std::vector<double> h(np);
+	const hipError_t e = hipMemcpy(h.data(), dp, np * sizeof(double), hipMemcpyDeviceToHost);
+	(void)hipFree(dp);
+	if (e != hipSuccess)
+		return BACCHUS_FAIL("project: %s", hipGetErrorString(e));
+	for (int64_t i = 0; i < n; i++)
+		for (int64_t b = 0; b < B; b++) {
+			double s = 0;
+			for (int c = 0; c < kChunks; c++)
+				s += h[(i * kChunks + c) * B + b];
+			out[i * B + b] = s;
+		}
+	return 0;
+}
+
+int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D, const float *c, double *out,
+			      void *stream)
+{
+	if (!U || !c || !out || n <= 0 || D <= 0)
+		return BACCHUS_FAIL("bad arguments");
+	hipStream_t s = (hipStream_t)stream;
+	hipLaunchKernelGGL(combine_basis, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, s, U, (int)n,
+			   D, c, out);
+	hipLaunchKernelGGL(normalize_vector, dim3(1), dim3(256), 0, s, out, D);
+	BACCHUS_HIP_CHECK(hipGetLastError());
+	return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""Learned Lanczos start for the family H(c1, c2) = H1 + c1 V2 + c2 V3.
+
+Pipeline (no counterpart in the reference, whose Lanczos starts from e_0,
+src/Bacchus/lanczos/lanczos.c:60-72):
+
+1. :func:`solve_family` runs the batched device Lanczos over many (c1, c2)
+   points at once (one batched matvec per iteration for all of them) and
+   returns, per point, its lowest eigenvalues and a *target vector*: the
+   ground state (``target="ground"``, sign fixed against a reference vector)
+   or the projection of a fixed vector onto the lowest ``m`` eigenvectors
+   (``target="band"``, free of the eigenvectors' sign and of their order
+   inside the band).
+2. :func:`fit` keeps ``n`` of the targets as the stored FP32 basis U, turns
+   every target into least-squares coefficients over U, and trains the MLP
+   (our BF16-MFMA forward/backward and Adam kernels) from (c1, c2) to those
+   coefficients.
+3. :meth:`LearnedStart.start_vector` evaluates the MLP and combines the
+   basis into a normalized FP64 start vector on the device; the Lanczos
+   solve then uses it with ``START_GIVEN``.  A random start is the fallback
+   when the prediction is not usable (non-finite or zero).
+
+Device tensors come from torch (memory and streams only); every arithmetic
+step on the path runs in the library's HIP kernels.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import bacchus
+
+
+def family_points(count, seed, low=0.8, high=1.2):
+    """(count, 2) uniform (c1, c2) draws."""
+    return np.random.default_rng(seed).uniform(low, high, size=(count, 2))
+
+
+def scales_of(points):
+    s = np.ones((len(points), 3))
+    s[:, 1:] = points
+    return s
+
+
+@dataclass
+class FamilySolves:
+    points: np.ndarray          # (P, 2)
+    eigenvalues: np.ndarray     # (P, nev)
+    targets: object             # torch float32 (P, D), unit norm
+    iterations: int
+    seconds: float
+
+
+def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64, target="ground",
+                 band=5, reference=None, seed=1, keep_targets=True, tolerance=0.0):
+    """Batched Lanczos over the points with an FP32/BF16 operator.
+
+    reference: float64 device vector (D,) that fixes the ground state's
+    sign / is projected onto the band.  Default: the ground state of the
+    first point (ground target; its overlap with every other ground state of
+    a smooth family stays far from zero, so the sign is continuous in c),
+    or a seeded random vector (band target)."""
+    import torch
+    D = operator.dimension
+    if reference is None and target != "ground":
+        g = torch.Generator(device="cuda")
+        g.manual_seed(int(seed) + 12345)
+        reference = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+        reference /= reference.norm()
+    nritz = 1 if target == "ground" else int(band)
+    evs, tg = [], []
+    t0 = time.time()
+    total_it = 0
+    for start in range(0, len(points), batch):
+        pts = points[start:start + batch]
+        B = len(pts)
+        with bacchus.Batch([operator], [B]) as bt:
+            bt.set_scales(0, scales_of(pts))
+            ritz = torch.empty((nritz, D, B), dtype=torch.float64, device="cuda")
+            ev, it, _ = bt.lanczos(iterations, num_eigenvalues=num_eigenvalues,
+                                   tolerance=tolerance, seed=seed + start, ritz_vectors=[ritz],
+                                   num_ritz=nritz)
+        total_it += it
+        evs.append(ev[0])
+        if keep_targets:
+            if reference is None:
+                reference = ritz[0][:, 0].clone()
+            # overlaps of every Ritz vector with the reference, per instance
+            ov = torch.einsum("rdb,d->rb", ritz, reference)
+            if target == "ground":
+                t = ritz[0] * torch.sign(ov[0])[None, :]
+            else:
+                t = torch.einsum("rdb,rb->db", ritz, ov)
+            t = t / t.norm(dim=0, keepdim=True)
+            tg.append(t.T.to(torch.float32).contiguous())
+        del ritz
+    targets = torch.cat(tg, 0) if keep_targets else None
+    torch.cuda.synchronize()
+    return FamilySolves(np.asarray(points), np.concatenate(evs, 0), targets, total_it,
+                        time.time() - t0)
+
+
+@dataclass
+class LearnedStart:
+    basis: object               # torch float32 (n, D)
+    mlp: bacchus.MLP
+    out_mean: np.ndarray        # (n,)
+    out_scale: float
+    train_loss: float
+    target: str
+
+    def coefficients(self, points):
+        import torch
+        x = torch.as_tensor((np.asarray(points, dtype=np.float64).reshape(-1, 2) - 1.0) / 0.2,
+                            dtype=torch.float32, device="cuda").contiguous()
+        y = self.mlp.forward(x)
+        m = torch.as_tensor(self.out_mean, dtype=torch.float32, device="cuda")
+        return (y * self.out_scale + m).contiguous()
+
+    def start_vector(self, c1, c2, out=None, mix=0.05, seed=11):
+        """Normalized FP64 start vector for H(c1, c2) on the device, or None
+        when the prediction is unusable (the caller then starts randomly).
+        A random component of relative size `mix` keeps every eigenvector
+        in the Krylov space, so a prediction that lands on the wrong state
+        (near a level crossing) still converges to the right one."""
+        import torch
+        c = self.coefficients([[c1, c2]])[0].contiguous()
+        if not bool(torch.isfinite(c).all()) or float(c.abs().max()) == 0.0:
+            return None
+        v = bacchus.basis_combine(self.basis, c, out)
+        if mix > 0:
+            g = torch.Generator(device="cuda")
+            g.manual_seed(seed)
+            r = torch.randn(v.shape, dtype=torch.float64, device="cuda", generator=g)
+            v += (mix / float(r.norm())) * r
+            v /= v.norm()
+        return v
+
+
+def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
+        ridge=1e-8, seed=1, target="ground", log=None):
+    """Stored basis = the first basis_size targets; MLP trained on the rest
+    (all points) to the least-squares coefficients over that basis."""
+    import torch
+    T = solves.targets
+    U = T[:basis_size].contiguous()
+    n = U.shape[0]
+    G = bacchus.basis_gram(U)
+    G += ridge * np.trace(G) / n * np.eye(n)
+    # coefficients of every target: (U U^T) z = U t, in chunks of 64 vectors
+    Z = []
+    for s in range(0, T.shape[0], 64):
+        V = T[s:s + 64].to(torch.float64).T.contiguous()       # (D, b)
+        Z.append(np.linalg.solve(G, bacchus.basis_project(U, V)).T)
+    Z = np.concatenate(Z, 0)                                     # (P, n)
+    mean = Z.mean(0)
+    scale = float(np.abs(Z - mean).max()) or 1.0
+    X = (solves.points - 1.0) / 0.2
+    xd = torch.as_tensor(X, dtype=torch.float32, device="cuda").contiguous()
+    td = torch.as_tensor((Z - mean) / scale, dtype=torch.float32, device="cuda").contiguous()
+    mlp = bacchus.MLP(2, hidden, depth, n, seed=seed)
+    loss = None
+    for k in range(steps):
+        want = (k + 1) % 500 == 0 or k + 1 == steps
+        step_lr = lr * (0.1 if k >= 0.75 * steps else 1.0)
+        loss = mlp.train_step(xd, td, lr=step_lr, want_loss=want) if want else \
+            mlp.train_step(xd, td, lr=step_lr)
+        if want and log:
+            log(f"step {k + 1} loss {loss:.3e}")
+    torch.cuda.synchronize()
+    return LearnedStart(U, mlp, mean, scale, float(loss), target)
+
+
+def solve_from(operator, c, start_vector, *, num_eigenvalues, targets, tolerance=1e-6,
+               max_iterations=300, seed=7):
+    """Single-vector Lanczos for H(c) from the given start (random if None)
+    until the lowest num_eigenvalues are within tolerance of targets."""
+    operator.set_scales(1.0, float(c[0]), float(c[1]))
+    kw = dict(num_eigenvalues=num_eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
+              convergence=bacchus.CONVERGE_TARGETS, target_eigenvalues=targets,
+              tolerance=tolerance, check_every=1)
+    if start_vector is None:
+        return bacchus.diagonalize(operator, max_iterations, start=bacchus.START_RANDOM,
+                                   seed=seed, **kw)
+    return bacchus.diagonalize(operator, max_iterations, start=bacchus.START_GIVEN,
+                               start_vector=start_vector, **kw)
