@@ -44,25 +44,32 @@ struct TInstr {
 	int32_t pad;
 };
 
-// One output tile: rows [po_begin, po_end) of a basis block (all neutron
-// states) and the range of virtual instructions summed into it.
+// One output tile: rows [po_begin, po_end) of a basis block (all columns)
+// and the range of virtual instructions summed into it.  In the normal
+// space a block is [proton][neutron] (rows = proton states, columns =
+// neutron states); in the transposed space (space = 1) it is
+// [neutron][proton], read from and written to the transposed copies of x
+// and y, and proton and neutron lists exchange their roles.
 struct TileWork {
 	int64_t y_off;     // offset of the block in y
-	int32_t nn_out;    // neutron states of the block
+	int32_t nn_out;    // columns of the block (neutron states; protons in
+			   // the transposed space)
 	int32_t po_begin;
 	int32_t po_end;
 	int32_t vi_begin;
 	int32_t vi_end;
-	int32_t shared;    // other tiles cover the same rows: flush atomically
+	int32_t shared;    // 1: other tiles cover the same rows, flush
+			   // atomically; 2: direct tile (N tasks only, no LDS
+			   // tile, sums added to y atomically)
 	int32_t t_begin;   // tasks [t_begin, t_end)
 	int32_t t_end;
-	int32_t pad;
+	int32_t space;     // 0 normal, 1 transposed
 };
 
-// One unit of a tile's work: entries [jb, je) of an N instruction over all
-// rows (r = -1), or proton entries [jb, je) (rows of the tile, in row
-// order) of a P/NP instruction with the range [e0, e1) of neutron entries
-// (NP) or neutron states (P) it covers.
+// One unit of a tile's work: CSR rows [jb, je) (output columns) of an N
+// instruction over all rows of the tile (r = -1), or row-side entries
+// [jb, je) (rows of the tile, in row order) of a P/NP instruction with the
+// range [e0, e1) of lane-side entries (NP) or columns (P) it covers.
 struct Task {
 	int32_t vi;
 	int32_t r;
@@ -78,6 +85,11 @@ constexpr int kTileThreads = 256;
 constexpr int kTileMaxElems = 2048;  // doubles of LDS per tile (16 KiB)
 constexpr uint32_t kNTaskEntries = 2048;  // list entries per N task
 constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
+// P instructions of blocks with fewer neutron states than this run in the
+// transposed space (as N instructions there)
+constexpr int64_t kWideRows = 64;
+// rows of a direct tile (N instructions, register sums, atomics into y)
+constexpr int64_t kDirectRows = 32;
 
 // Tiling of one operator's output: the tiles (rows of a block x all
 // neutron states x bt vectors) with their tasks, heaviest first.
@@ -122,6 +134,13 @@ struct bacchus_hip_operator {
 	bacchus::Task *d_tasks = nullptr;
 	int64_t num_tasks = 0;
 	float *d_xf = nullptr;          // FP32 copy of x (FP32/BF16 modes)
+	// transposed space: per-block [neutron][proton] copies of x and y
+	bool has_transposed = false;
+	void *d_xT = nullptr;           // storage-precision vector type
+	double *d_yT = nullptr;
+	int64_t *d_blk_off = nullptr;   // block offsets, proton and neutron
+	int32_t *d_blk_np = nullptr;    // counts (device copies)
+	int32_t *d_blk_nn = nullptr;
 	int64_t num_tiles = 0;
 	int64_t num_tinstr = 0;
 	size_t tile_lds_bytes = 0;
@@ -142,4 +161,46 @@ struct bacchus_hip_operator {
 
 namespace bacchus {
 int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out);
+
+constexpr int kTransposeSplits = 8;
+
+// xT[b][n][p] = x[b][p][n] for every block (B interleaved vectors; one
+// workgroup row per block, grid.y splits a block)
+template <typename T>
+__global__ void transpose_blocks(const T *__restrict__ x, T *__restrict__ xT,
+				 const int64_t *__restrict__ off, const int32_t *__restrict__ np,
+				 const int32_t *__restrict__ nn, int B)
+{
+	const int blk = blockIdx.x;
+	const int64_t o = off[blk];
+	const int P = np[blk], N = nn[blk];
+	const int64_t total = (int64_t)P * N * B;
+	for (int64_t t = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; t < total;
+	     t += (int64_t)blockDim.x * gridDim.y) {
+		const int64_t e = t / B;
+		const int b = (int)(t - e * B);
+		const int p = (int)(e / N), n = (int)(e - (int64_t)p * N);
+		xT[(o + (int64_t)n * P + p) * B + b] = x[(o + e) * B + b];
+	}
+}
+
+// y[b][p][n] += yT[b][n][p] for every block
+template <typename T>
+__global__ void add_transposed_blocks(T *__restrict__ y, const T *__restrict__ yT,
+				      const int64_t *__restrict__ off, const int32_t *__restrict__ np,
+				      const int32_t *__restrict__ nn, int B)
+{
+	const int blk = blockIdx.x;
+	const int64_t o = off[blk];
+	const int P = np[blk], N = nn[blk];
+	const int64_t total = (int64_t)P * N * B;
+	for (int64_t t = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; t < total;
+	     t += (int64_t)blockDim.x * gridDim.y) {
+		const int64_t e = t / B;
+		const int b = (int)(t - e * B);
+		const int p = (int)(e / N), n = (int)(e - (int64_t)p * N);
+		y[(o + e) * B + b] += yT[(o + (int64_t)n * P + p) * B + b];
+	}
+}
+
 }

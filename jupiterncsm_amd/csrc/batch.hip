@@ -30,11 +30,14 @@ struct BatchOp {
 	const TileWork *tiles;
 	const Task *tasks;
 	const TInstr *ins;
+	const uint32_t *rowptr;
 	const uint2 *ent;
 	const uint32_t *keys;
 	const void *V;
 	const float *x;
 	float *y;
+	const float *xT;      // per-block transposed copies ([neutron][proton])
+	float *yT;
 	const float *scales;  // [B][4], index 1..3 = force class
 	int32_t B;
 	int32_t precision;
@@ -78,7 +81,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int bt = 1 << lbt;
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
-	const int twords = (nrow * nn) << lbt;
+	const bool direct = w.shared == 2;  // N only, sums go straight to y
+	const int twords = direct ? 0 : (nrow * nn) << lbt;
+	float *ys = w.space ? o.yT : o.y;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x)
 		tile[t] = 0.f;
 	__syncthreads();
@@ -92,12 +97,13 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int b = b0 + bl;
 	const bool bvalid = b < B;
 	const VT *V = (const VT *)o.V;
+	const float *xs = w.space ? o.xT : o.x;
 	for (int32_t ti = w.t_begin + wave; ti < w.t_end; ti += nwaves) {
 		const Task T = o.tasks[ti];
 		const TInstr I = o.ins[T.vi];
 		const float sc = bvalid ? o.scales[b * 4 + I.force] : 0.f;
 		// lane's column of the input block: element e at xb[e * B]
-		const float *xb = o.x + (size_t)I.x_off * B + (bvalid ? b : 0);
+		const float *xb = xs + (size_t)I.x_off * B + (bvalid ? b : 0);
 		const uint32_t nn_in = (uint32_t)I.nn_in;
 		if (I.type == 3) {
 			const VT *vblk = V + I.v_off;
@@ -207,24 +213,63 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					atomicAdd(&tile[(((cur - w.po_begin) * nn + n) << lbt) + bl], sc * (a0 + a1));
 			}
 		} else {
-			for (uint32_t k0 = T.jb; k0 < T.je; k0 += G) {
-				const uint32_t k = k0 + slot;
-				const bool valid = k < T.je && bvalid;
-				const uint32_t kk = k < T.je ? k : T.jb;
-				const uint2 en = o.ent[kk];
-				float v = Load<VT>::at(V, (size_t)I.v_off + (en.y & kMask));
-				if (!valid || fabsf(v) < 1e-12f)
+			// CSR rows [jb, je) (output columns): one column at a time,
+			// slots striding its entries, sums for up to 8 tile rows in
+			// registers, then a reduction over the slots of each vector
+			const float *xrow0 = xb + (size_t)nn_in * w.po_begin * B;
+			for (uint32_t col = T.jb; col < T.je; col++) {
+				const uint32_t kb = o.rowptr[I.n_base + col];
+				const uint32_t ke = o.rowptr[I.n_base + col + 1];
+				if (kb == ke)
 					continue;
-				v = sign_of(v, en.y) * sc;
-				const int no = (int)o.keys[kk];
-				const float *xc = xb + ((size_t)nn_in * w.po_begin + en.x) * B;
-				for (int r = 0; r < nrow; r++)
-					atomicAdd(&tile[((r * nn + no) << lbt) + bl], v * xc[(size_t)nn_in * r * B]);
+				for (int r0 = 0; r0 < nrow; r0 += 8) {
+					float acc[8];
+#pragma unroll
+					for (int q = 0; q < 8; q++)
+						acc[q] = 0.f;
+					for (uint32_t k = kb + slot; k < ke; k += 4 * G) {
+						uint2 en[4];
+						float v[4];
+#pragma unroll
+						for (int u = 0; u < 4; u++)
+							en[u] = k + G * u < ke ? o.ent[k + G * u] : make_uint2(0u, 0u);
+#pragma unroll
+						for (int u = 0; u < 4; u++) {
+							const float e = Load<VT>::at(V, (size_t)I.v_off + (en[u].y & kMask));
+							v[u] = (k + G * u < ke && !(fabsf(e) < 1e-12f)) ? sign_of(e, en[u].y)
+													 : 0.f;
+						}
+#pragma unroll
+						for (int u = 0; u < 4; u++) {
+							const float *xc = xrow0 + ((size_t)nn_in * r0 + en[u].x) * B;
+#pragma unroll
+							for (int q = 0; q < 8; q++)
+								if (r0 + q < nrow)
+									acc[q] += v[u] * xc[(size_t)nn_in * q * B];
+						}
+					}
+#pragma unroll
+					for (int q = 0; q < 8; q++) {
+						if (r0 + q >= nrow)
+							break;
+						float v = acc[q];
+						for (int off = 32; off >= bt; off >>= 1)
+							v += __shfl_xor(v, off, 64);
+						if (slot == 0 && bvalid && v != 0.f) {
+							if (direct)
+								atomicAdd(ys + ((size_t)w.y_off +
+										(size_t)(w.po_begin + r0 + q) * nn + col) * B + b,
+									  sc * v);
+							else
+								atomicAdd(&tile[(((r0 + q) * nn + (int)col) << lbt) + bl], sc * v);
+						}
+					}
+				}
 			}
 		}
 	}
 	__syncthreads();
-	float *yblk = o.y + ((size_t)w.y_off + (size_t)nn * w.po_begin) * B + b0;
+	float *yblk = ys + ((size_t)w.y_off + (size_t)nn * w.po_begin) * B + b0;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x) {
 		const int e = t >> lbt, c = t & (bt - 1);
 		if (b0 + c >= B)
@@ -311,6 +356,9 @@ struct bacchus_hip_batch {
 	int grid = 0;
 	// internal vector buffers (autotuning, Lanczos)
 	std::vector<float *> d_x, d_y;
+	// per-operator transposed copies of x and y (operators with
+	// transposed-space instructions)
+	std::vector<float *> d_xT, d_yT;
 	double bytes = 0, flops = 0;
 
 	void release_plan()
@@ -332,6 +380,10 @@ struct bacchus_hip_batch {
 		for (auto *p : d_x)
 			(void)hipFree(p);
 		for (auto *p : d_y)
+			(void)hipFree(p);
+		for (auto *p : d_xT)
+			(void)hipFree(p);
+		for (auto *p : d_yT)
 			(void)hipFree(p);
 		(void)hipFree(d_ops);
 		(void)hipFree(d_next);
@@ -376,14 +428,19 @@ int build_plan(bacchus_hip_batch_t *bt)
 						    (pow2_floor(bt->B[i]) < bt->B[i] ? 2 : 1));
 		PlanParams pp;
 		pp.target = target;
-		pp.tile_elems = std::max<int64_t>(words, *std::max_element(op->block_nn.begin(),
-									    op->block_nn.end()));
-		pp.bt.resize(nb);
-		for (int64_t b = 0; b < nb; b++) {
+		// row widths: neutron states (normal space), proton states
+		// (transposed space)
+		int64_t widest = 1;
+		for (int64_t b = 0; b < nb; b++)
+			widest = std::max({widest, op->block_nn[b], op->block_np[b]});
+		pp.tile_elems = std::max<int64_t>(words, widest);
+		pp.bt.resize(2 * nb);
+		for (int64_t slot = 0; slot < 2 * nb; slot++) {
+			const int64_t width = slot < nb ? op->block_nn[slot] : op->block_np[slot - nb];
 			int v = vec;
-			while (v > 1 && op->block_nn[b] * v > pp.tile_elems)
+			while (v > 1 && width * v > pp.tile_elems)
 				v >>= 1;
-			pp.bt[b] = v;
+			pp.bt[slot] = v;
 		}
 		Plan plan;
 		if (int rc = make_plan(op, pp, plan))
@@ -433,13 +490,29 @@ int build_plan(bacchus_hip_batch_t *bt)
 int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 {
 	for (size_t i = 0; i < bt->ops.size(); i++) {
-		const int64_t n = bt->ops[i]->dimension * bt->B[i];
+		const bacchus_hip_operator_t *op = bt->ops[i];
+		const int64_t n = op->dimension * bt->B[i];
 		hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->h_ops[i].y, n, 0.f);
+		if (op->has_transposed) {
+			hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_yT[i], n, 0.f);
+			hipLaunchKernelGGL(transpose_blocks<float>,
+					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
+					   bt->h_ops[i].x, bt->d_xT[i], op->d_blk_off, op->d_blk_np,
+					   op->d_blk_nn, bt->B[i]);
+		}
 	}
 	if (bt->num_items) {
 		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next);
 		hipLaunchKernelGGL(batch_kernel, dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
 				   bt->d_ops, bt->d_items, (int)bt->num_items, bt->d_next);
+	}
+	for (size_t i = 0; i < bt->ops.size(); i++) {
+		const bacchus_hip_operator_t *op = bt->ops[i];
+		if (op->has_transposed)
+			hipLaunchKernelGGL(add_transposed_blocks<float>,
+					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
+					   bt->h_ops[i].y, bt->d_yT[i], op->d_blk_off, op->d_blk_np,
+					   op->d_blk_nn, bt->B[i]);
 	}
 	BACCHUS_HIP_CHECK(hipGetLastError());
 	return 0;
@@ -495,11 +568,12 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		if (ops[i]->dimension * (int64_t)num_vectors[i] > (int64_t)INT32_MAX)
 			return BACCHUS_FAIL("operator %d: D x B = %lld exceeds 2^31", i,
 					    (long long)(ops[i]->dimension * num_vectors[i]));
-		for (int64_t nn : ops[i]->block_nn)
-			if (nn > kBatchTileWords)
-				return BACCHUS_FAIL("operator %d has a block of %lld neutron states; "
-						    "the batched tile holds at most %lld", i,
-						    (long long)nn, (long long)kBatchTileWords);
+		for (size_t b = 0; b < ops[i]->block_nn.size(); b++)
+			if (std::max(ops[i]->block_nn[b], ops[i]->block_np[b]) > kBatchTileWords)
+				return BACCHUS_FAIL("operator %d has a block of %lld x %lld states; the "
+						    "batched tile holds rows of at most %lld", i,
+						    (long long)ops[i]->block_np[b], (long long)ops[i]->block_nn[b],
+						    (long long)kBatchTileWords);
 	}
 	auto *bt = new bacchus_hip_batch;
 	if (tuning)
@@ -508,6 +582,8 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 	bt->B.assign(num_vectors, num_vectors + num_ops);
 	bt->d_x.assign(num_ops, nullptr);
 	bt->d_y.assign(num_ops, nullptr);
+	bt->d_xT.assign(num_ops, nullptr);
+	bt->d_yT.assign(num_ops, nullptr);
 	bt->h_ops.resize(num_ops);
 	auto fail = [&](int rc) {
 		delete bt;
@@ -523,9 +599,18 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		bt->d_scales.push_back(ds);
 		if (hipMemcpy(ds, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
 			return fail(BACCHUS_FAIL("upload of the scales failed"));
+		if (op->has_transposed) {
+			const size_t n = (size_t)op->dimension * B * sizeof(float);
+			if (hipMalloc(&bt->d_xT[i], n) != hipSuccess ||
+			    hipMalloc(&bt->d_yT[i], n) != hipSuccess)
+				return fail(BACCHUS_FAIL("hipMalloc of the transposed vectors failed"));
+		}
 		BatchOp &o = bt->h_ops[i];
 		o = BatchOp{};
 		o.ins = op->d_tinstr;
+		o.rowptr = op->d_rowptr;
+		o.xT = bt->d_xT[i];
+		o.yT = bt->d_yT[i];
 		o.ent = op->d_entries;
 		o.keys = op->d_keys;
 		o.V = op->d_v;

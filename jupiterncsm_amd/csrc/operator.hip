@@ -79,18 +79,22 @@ template <> struct Elem<uint16_t> {
 };
 
 // Owner-computes tile kernel.  A workgroup owns an output tile: the rows
-// p_o in [po_begin, po_end) of one basis block, all neutron states, held in
-// LDS.  It applies every virtual instruction that writes the block (the
+// p_o in [po_begin, po_end) of one basis block, all columns, held in LDS.
+// It applies every virtual instruction that writes the block (the
 // instruction itself, or for off-diagonal instructions its transpose,
-// scheduler.c:310-418) and writes the tile to y once.
+// scheduler.c:310-418) and writes the tile to y once.  Tiles of the
+// transposed space read xT and write yT (blocks stored [neutron][proton]),
+// with the proton and neutron lists in exchanged roles.
 //   NP (c.f. matrix_vector_multiplication.c:134-210): for each row p_o,
-//     threads over neutron entries i (ordered by element index):
+//     threads over lane-side entries i (ordered by element index):
 //     tile[p_o][n_o(i)] += s_i sum_{j in Lp(p_o)} s_j W[c_i + |C_n| c_j] x[n_i + Nn_in p_j]
 //     with W the instruction's dense element sub-block and c the compact
 //     element indices (see bacchus_hip_operator_create)
-//   N  (:20-77): thread per tile element, CSR row of n_o;
+//   N  (:20-77): 16-lane segments per output column n_o (a CSR row of the
+//     list ordered by output index), lanes striding its entries, sums for
+//     every tile row in registers, then a segment reduction:
 //     tile[p_o][n_o] += sum_{i in Ln(n_o)} s_i V[m_i] x[n_i + Nn_in p_o]
-//   P  (:79-132): thread per tile element, CSR row of p_o;
+//   P  (:79-132): lanes over columns n, proton entries broadcast;
 //     tile[p_o][n_o] += sum_{j in Lp(p_o)} s_j V[m_j] x[n_o + Nn_in p_j]
 // N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
@@ -99,17 +103,21 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 	const TInstr *__restrict__ ins,
 	const uint32_t *__restrict__ rowptr, const uint2 *__restrict__ ent,
 	const uint32_t *__restrict__ keys, const VT *__restrict__ V,
-	const typename Elem<VT>::x_t *__restrict__ x, double *__restrict__ y, double s1,
-	double s2, double s3)
+	const typename Elem<VT>::x_t *__restrict__ x_normal,
+	const typename Elem<VT>::x_t *__restrict__ x_transposed, double *__restrict__ y_normal,
+	double *__restrict__ y_transposed, double s1, double s2, double s3)
 {
 	using acc_t = typename Elem<VT>::acc_t;
 	extern __shared__ double tile[];
 	// tiles are sorted heaviest first and dispatched in that order
 	const TileWork w = work[blockIdx.x];
 	(void)nwork;
+	const auto *__restrict__ x = w.space ? x_transposed : x_normal;
+	double *__restrict__ y = w.space ? y_transposed : y_normal;
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
-	const int telems = nrow * nn;
+	const bool direct = w.shared == 2;  // N only, sums go straight to y
+	const int telems = direct ? 0 : nrow * nn;
 	for (int t = threadIdx.x; t < telems; t += blockDim.x)
 		tile[t] = 0.0;
 	__syncthreads();
@@ -245,24 +253,64 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					atomicAdd(&tile[(cur - w.po_begin) * nn + n], (double)(sc * (a0 + a1)));
 			}
 		} else {
-			// entries [jb, je) of the neutron list copy ordered by output
-			// row: lanes over entries, each applied to every row of the
-			// tile (the protons are spectators)
-			const auto *xblk = x + I.x_off;
+			// CSR rows [jb, je) of the list ordered by output column: a
+			// 16-lane segment per column, lanes striding its entries (the
+			// entry and its element are loaded once for every tile row);
+			// up to 8 tile rows per pass in registers
+			const auto *xblk = x + I.x_off + (size_t)I.nn_in * w.po_begin;
 			const uint32_t nn_in = (uint32_t)I.nn_in;
-			for (uint32_t k0 = T.jb; k0 < T.je; k0 += 64) {
-				const uint32_t k = k0 + lane;
-				const bool valid = k < T.je;
-				const uint32_t kk = valid ? k : T.jb;
-				const uint2 en = ent[kk];
-				acc_t v = Elem<VT>::load(V, I.v_off + (en.y & kIndexMask));
-				if (!valid || fabs(v) < acc_t(1e-12))
-					continue;
-				v = apply_sign(v, en.y) * sc;
-				const uint32_t no = keys[kk];
-				const auto *xc = xblk + en.x + (size_t)nn_in * w.po_begin;
-				for (int r = 0; r < nrow; r++)
-					atomicAdd(&tile[r * nn + no], (double)(v * acc_t(xc[(size_t)nn_in * r])));
+			const int seg = lane >> 4, sl = lane & 15;
+			for (uint32_t c0 = T.jb; c0 < T.je; c0 += 4) {
+				const uint32_t col = c0 + seg;
+				const bool cv = col < T.je;
+				const uint32_t kb = cv ? rowptr[I.n_base + col] : 0;
+				const uint32_t ke = cv ? rowptr[I.n_base + col + 1] : 0;
+				for (int r0 = 0; r0 < nrow; r0 += 8) {
+					acc_t acc[8];
+#pragma unroll
+					for (int q = 0; q < 8; q++)
+						acc[q] = 0;
+					// four entries per lane in flight: their loads are
+					// issued together before any product needs them
+					for (uint32_t k = kb + sl; k < ke; k += 64) {
+						uint2 en[4];
+						acc_t v[4];
+#pragma unroll
+						for (int u = 0; u < 4; u++)
+							en[u] = k + 16 * u < ke ? ent[k + 16 * u] : make_uint2(0u, 0u);
+#pragma unroll
+						for (int u = 0; u < 4; u++) {
+							const acc_t e = Elem<VT>::load(V, I.v_off + (en[u].y & kIndexMask));
+							v[u] = (k + 16 * u < ke && !(fabs(e) < acc_t(1e-12)))
+								       ? apply_sign(e, en[u].y) : acc_t(0);
+						}
+#pragma unroll
+						for (int u = 0; u < 4; u++) {
+							const auto *xc = xblk + en[u].x + (size_t)nn_in * r0;
+#pragma unroll
+							for (int q = 0; q < 8; q++)
+								if (r0 + q < nrow)
+									acc[q] += v[u] * acc_t(xc[(size_t)nn_in * q]);
+						}
+					}
+#pragma unroll
+					for (int q = 0; q < 8; q++) {
+						if (r0 + q >= nrow)
+							break;
+						acc_t v = acc[q];
+						v += __shfl_xor(v, 8, 64);
+						v += __shfl_xor(v, 4, 64);
+						v += __shfl_xor(v, 2, 64);
+						v += __shfl_xor(v, 1, 64);
+						if (sl == 0 && cv && v != acc_t(0)) {
+							if (direct)
+								atomicAdd(y + w.y_off + (int64_t)(w.po_begin + r0 + q) * nn + col,
+									  (double)(sc * v));
+							else
+								atomicAdd(&tile[(r0 + q) * nn + col], (double)(sc * v));
+						}
+					}
+				}
 			}
 		}
 	}
@@ -294,12 +342,13 @@ __global__ void to_float_kernel(const double *x, float *xf, int64_t n)
 
 template <typename VT>
 void launch(const bacchus_hip_operator *op, const VT *V, const typename Elem<VT>::x_t *x,
-	    double *y, double s1, double s2, double s3, hipStream_t s)
+	    const typename Elem<VT>::x_t *xT, double *y, double s1, double s2, double s3,
+	    hipStream_t s)
 {
 	if (op->num_tiles)
 		hipLaunchKernelGGL(tile_kernel<VT>, dim3(op->num_tiles), dim3(kTileThreads),
 				   op->tile_lds_bytes, s, op->d_tiles, op->num_tiles, op->d_tasks, op->d_tinstr,
-				   op->d_rowptr, op->d_entries, op->d_keys, V, x, y, s1, s2, s3);
+				   op->d_rowptr, op->d_entries, op->d_keys, V, x, xT, y, op->d_yT, s1, s2, s3);
 }
 
 uint16_t to_bf16(double v)
@@ -353,14 +402,79 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	const int64_t nb = (int64_t)op->block_np.size();
 	out = Plan();
 	std::vector<int> tile_bt;
-	for (int64_t b = 0; b < nb; b++) {
-		const int32_t first = op->blk_vi[b], last = op->blk_vi[b + 1];
+	std::vector<Task> direct_tasks;
+	for (int64_t slot = 0; slot < 2 * nb; slot++) {
+		const int32_t first = op->blk_vi[slot];
+		int32_t last = op->blk_vi[slot + 1];
 		if (first == last)
 			continue;
-		const int64_t nn = op->block_nn[b], np = op->block_np[b];
-		const int bt = pp.bt.empty() ? 1 : pp.bt[b];
+		const int64_t b = slot % nb;
+		const int space = slot >= nb;
+		// rows x columns of the block in its space
+		const int64_t nn = space ? op->block_np[b] : op->block_nn[b];
+		const int64_t np = space ? op->block_nn[b] : op->block_np[b];
+		const int bt = pp.bt.empty() ? 1 : pp.bt[slot];
+		// N instructions (at the end of the slot's range): direct tiles of
+		// up to kDirectRows rows and a range of output columns; their sums
+		// go straight to y with atomics, no LDS tile
+		int32_t nfirst = first;
+		while (nfirst < last && tinstr[nfirst].type != 1)
+			nfirst++;
+		for (int32_t v = nfirst; v < last; v++) {
+			const TInstr &I = tinstr[v];
+			for (int64_t po0 = 0; po0 < np; po0 += kDirectRows) {
+				const int64_t po1 = std::min<int64_t>(np, po0 + kDirectRows);
+				const double R = double(po1 - po0);
+				const uint64_t want = std::max<uint64_t>(
+					64, pp.task_mults / std::max<uint64_t>(1, (uint64_t)R * bt));
+				int64_t c0 = 0;
+				while (c0 < nn) {
+					// columns of one tile: about target multiplications
+					int64_t c1 = c0;
+					double cost = 0;
+					while (c1 < nn && (cost == 0 || cost < pp.target)) {
+						cost += double(rowptr[I.n_base + c1 + 1] - rowptr[I.n_base + c1]) *
+							(R + 2.0) * bt;
+						c1++;
+					}
+					TileWork w{};
+					w.y_off = op->block_off[b];
+					w.nn_out = (int32_t)nn;
+					w.po_begin = (int32_t)po0;
+					w.po_end = (int32_t)po1;
+					w.vi_begin = v;
+					w.vi_end = v + 1;
+					w.shared = 2;
+					w.space = space;
+					w.t_begin = (int32_t)direct_tasks.size();
+					uint32_t r0 = (uint32_t)c0;
+					while (r0 < (uint32_t)c1) {
+						uint32_t r1 = r0 + 1;
+						while (r1 < (uint32_t)c1 &&
+						       rowptr[I.n_base + r1 + 1] - rowptr[I.n_base + r0] <= want)
+							r1++;
+						if (rowptr[I.n_base + r1] > rowptr[I.n_base + r0])
+							direct_tasks.push_back(Task{v, -1, r0, r1, 0, 0, 0, 0});
+						r0 = r1;
+					}
+					w.t_end = (int32_t)direct_tasks.size();
+					if (w.t_end > w.t_begin) {
+						out.tiles.push_back(w);
+						out.cost.push_back(cost);
+						tile_bt.push_back(bt);
+					}
+					c0 = c1;
+				}
+			}
+		}
+		// the direct tiles add into the same rows with atomics, so the LDS
+		// tiles of this block must flush atomically as well
+		const bool has_direct = nfirst < last;
+		last = nfirst;
+		if (first == last)
+			continue;
 		if (nn * bt > pp.tile_elems)
-			return BACCHUS_FAIL("basis block %lld has %lld neutron states; a tile "
+			return BACCHUS_FAIL("basis block %lld has rows of %lld states; a tile "
 					    "of %d vectors holds at most %lld", (long long)b,
 					    (long long)nn, bt, (long long)(pp.tile_elems / bt));
 		// cost of every (instruction, row) and of every row
@@ -398,11 +512,12 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 			w.nn_out = (int32_t)nn;
 			w.po_begin = (int32_t)po0;
 			w.po_end = (int32_t)po1;
+			w.space = space;
 			const int pieces = c > 2.0 * pp.target ? (int)std::ceil(c / pp.target) : 1;
 			if (pieces == 1) {
 				w.vi_begin = first;
 				w.vi_end = last;
-				w.shared = 0;
+				w.shared = has_direct ? 1 : 0;
 				out.tiles.push_back(w);
 				out.cost.push_back(c);
 				tile_bt.push_back(bt);
@@ -446,16 +561,33 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	out.bt = bts;
 	for (size_t ti = 0; ti < out.tiles.size(); ti++) {
 		TileWork &t = out.tiles[ti];
+		if (t.shared == 2) {
+			const int32_t b0 = t.t_begin, b1 = t.t_end;
+			t.t_begin = (int32_t)out.tasks.size();
+			out.tasks.insert(out.tasks.end(), direct_tasks.begin() + b0,
+					 direct_tasks.begin() + b1);
+			t.t_end = (int32_t)out.tasks.size();
+			continue;
+		}
 		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / bts[ti]);
 		t.t_begin = (int32_t)out.tasks.size();
 		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
 			const TInstr &I = tinstr[v];
 			if (I.type == 1) {
-				const uint32_t b = rowptr[I.n_base], e = rowptr[I.n_base + t.nn_out];
-				const uint32_t step = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(
-					kNTaskEntries, task_mults / std::max(1, t.po_end - t.po_begin)));
-				for (uint32_t k = b; k < e; k += step)
-					out.tasks.push_back(Task{v, -1, k, std::min(e, k + step), 0, 0, 0, 0});
+				// runs of CSR rows (output columns) holding about
+				// task_mults / rows entries each
+				const uint64_t want = std::max<uint64_t>(
+					64, task_mults / std::max(1, t.po_end - t.po_begin));
+				uint32_t r0 = 0;
+				while (r0 < (uint32_t)t.nn_out) {
+					uint32_t r1 = r0 + 1;
+					while (r1 < (uint32_t)t.nn_out &&
+					       rowptr[I.n_base + r1 + 1] - rowptr[I.n_base + r0] <= want)
+						r1++;
+					if (rowptr[I.n_base + r1] > rowptr[I.n_base + r0])
+						out.tasks.push_back(Task{v, -1, r0, r1, 0, 0, 0, 0});
+					r0 = r1;
+				}
 				continue;
 			}
 			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
@@ -502,6 +634,11 @@ void bacchus_hip_operator_free(bacchus_hip_operator_t *op)
 	(void)hipFree(op->d_tasks);
 	(void)hipFree(op->d_keys);
 	(void)hipFree(op->d_xf);
+	(void)hipFree(op->d_xT);
+	(void)hipFree(op->d_yT);
+	(void)hipFree(op->d_blk_off);
+	(void)hipFree(op->d_blk_np);
+	(void)hipFree(op->d_blk_nn);
 	delete op;
 }
 
@@ -563,7 +700,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	}
 	op->dimension = op->block_off[nb];
 	auto fail = [&](int code) {
-		delete op;
+		bacchus_hip_operator_free(op);
 		return code;
 	};
 
@@ -691,6 +828,23 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		const uint32_t c = (uint32_t)(std::lower_bound(mv.begin(), mv.end(), v) - mv.begin());
 		return c | ((uint32_t)m & 0x80000000u);
 	};
+	// --- which instructions run in the transposed space (every block read
+	// and written as [neutron][proton], protons fastest): NP instructions
+	// whose proton list is the longer one (the kernel's lanes then walk the
+	// long list, as for the others) and P instructions of blocks with few
+	// neutron states (they become N instructions there, whose lanes walk
+	// the list instead of the neutron states) ---
+	auto list_len = [&](int32_t l) -> int64_t {
+		return l >= 0 ? p->list_offsets[l + 1] - p->list_offsets[l] : 0;
+	};
+	std::vector<char> swapped(ni, 0);
+	for (int64_t i = 0; i < ni; i++) {
+		const bacchus_hip_instruction_t &I = p->instructions[i];
+		if (I.type == 3)
+			swapped[i] = list_len(I.proton_list) > list_len(I.neutron_list);
+		else if (I.type == 2)
+			swapped[i] = op->block_nn[I.block_out] < kWideRows;
+	}
 	std::map<std::tuple<int32_t, int32_t, int32_t>, int64_t> sub_of;
 	std::vector<int64_t> sub_off(ni, -1);
 	std::vector<double> W;
@@ -709,13 +863,22 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			for (uint32_t m : mvals[I.type == 1 ? I.neutron_list : I.proton_list])
 				W.push_back(Vb[m]);
 		} else {
+			// normal: rows over the proton set, columns over the neutron
+			// set; transposed space: the other way round.  The swap
+			// decision depends on the two lists only, so every
+			// instruction sharing the sub-block agrees with it.
 			const auto &mn = mvals[I.neutron_list], &mp = mvals[I.proton_list];
 			const int64_t Dn = vdn[I.matrix_block];
 			W.resize(off + (int64_t)mn.size() * (int64_t)mp.size());
 			double *dst = W.data() + off;
 			for (size_t b = 0; b < mp.size(); b++)
-				for (size_t a2 = 0; a2 < mn.size(); a2++)
-					dst[b * mn.size() + a2] = Vb[mn[a2] + Dn * (int64_t)mp[b]];
+				for (size_t a2 = 0; a2 < mn.size(); a2++) {
+					const double v = Vb[mn[a2] + Dn * (int64_t)mp[b]];
+					if (swapped[i])
+						dst[a2 * mp.size() + b] = v;
+					else
+						dst[b * mn.size() + a2] = v;
+				}
 		}
 		sub_of[key] = off;
 		sub_off[i] = off;
@@ -769,14 +932,17 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		}
 	}
 	op->list_entry_count = used_entries;
-	// neutron lists of NP instructions: a copy ordered by element index
-	// alone (rows kept in keys[])
+	// the lane-side list of every NP instruction (neutron list, or proton
+	// list in the transposed space): a copy ordered by element index alone
+	// (rows kept in keys[])
 	std::vector<int64_t> elem_by_out(nl, -1), elem_by_in(nl, -1);
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
-		if (I.type != 3 || elem_by_out[I.neutron_list] >= 0)
+		if (I.type != 3)
 			continue;
-		const int64_t l = I.neutron_list;
+		const int64_t l = swapped[i] ? I.proton_list : I.neutron_list;
+		if (elem_by_out[l] >= 0)
+			continue;
 		for (int dir = 0; dir < 2; dir++) {
 			const int64_t r0 = (dir == 0 ? rows_by_out : rows_by_in)[l];
 			const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
@@ -798,8 +964,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
 	}
 
-	// --- virtual instructions (each direction) grouped by output block ---
-	std::vector<std::vector<TInstr>> per_block(nb);
+	// --- virtual instructions (each direction) grouped by output block;
+	// slots [0, nb) are the normal space, [nb, 2 nb) the transposed one ---
+	std::vector<std::vector<TInstr>> per_block(2 * nb);
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		const int force = p->matrix_force ? p->matrix_force[I.matrix_block] : 2;
@@ -816,31 +983,53 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			TInstr t{};
 			t.x_off = op->block_off[bin];
 			t.v_off = sub_off[i];
-			t.dn = I.type == 3 ? (int64_t)mvals[I.neutron_list].size() : 0;
-			t.nn_in = (int32_t)op->block_nn[bin];
-			t.type = (int16_t)I.type;
 			t.force = (int16_t)std::min(3, std::max(1, force));
-			if (I.type == 3) {
-				t.n_base = (uint32_t)(dir == 0 ? elem_by_out : elem_by_in)[I.neutron_list];
-				t.n_len = (uint32_t)ln;
-			} else if (I.type == 1)
-				t.n_base = (uint32_t)rows[I.neutron_list];
-			if (I.type != 1)
-				t.p_rows = (uint32_t)rows[I.proton_list];
-			per_block[bout].push_back(t);
+			const auto &elem = dir == 0 ? elem_by_out : elem_by_in;
+			if (!swapped[i]) {
+				t.dn = I.type == 3 ? (int64_t)mvals[I.neutron_list].size() : 0;
+				t.nn_in = (int32_t)op->block_nn[bin];
+				t.type = (int16_t)I.type;
+				if (I.type == 3) {
+					t.n_base = (uint32_t)elem[I.neutron_list];
+					t.n_len = (uint32_t)ln;
+				} else if (I.type == 1)
+					t.n_base = (uint32_t)rows[I.neutron_list];
+				if (I.type != 1)
+					t.p_rows = (uint32_t)rows[I.proton_list];
+				per_block[bout].push_back(t);
+			} else {
+				// transposed space: protons fastest, the roles of the
+				// two lists exchanged
+				t.nn_in = (int32_t)op->block_np[bin];
+				if (I.type == 3) {
+					t.type = 3;
+					t.dn = (int64_t)mvals[I.proton_list].size();
+					t.n_base = (uint32_t)elem[I.proton_list];
+					t.n_len = (uint32_t)lp;
+					t.p_rows = (uint32_t)rows[I.neutron_list];
+				} else {
+					t.type = 1;
+					t.n_base = (uint32_t)rows[I.proton_list];
+				}
+				per_block[nb + bout].push_back(t);
+			}
 		}
 	}
 
 	// --- host copies for the planner; virtual instructions of a block
 	// sorted by element sub-block so neighbouring tasks share elements ---
-	op->blk_vi.assign(nb + 1, 0);
-	for (int64_t b = 0; b < nb; b++) {
+	op->blk_vi.assign(2 * nb + 1, 0);
+	for (int64_t b = 0; b < 2 * nb; b++) {
 		auto &vs = per_block[b];
+		// N instructions last (they form direct tiles), the others by
+		// element sub-block
 		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
-			return u.v_off < w.v_off;
+			return std::make_pair(u.type == 1, u.v_off) < std::make_pair(w.type == 1, w.v_off);
 		});
 		op->h_tinstr.insert(op->h_tinstr.end(), vs.begin(), vs.end());
 		op->blk_vi[b + 1] = (int32_t)op->h_tinstr.size();
+		if (b >= nb && !vs.empty())
+			op->has_transposed = true;
 	}
 	op->h_rowptr = rowptr;
 	op->mults = mults;
@@ -856,7 +1045,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	pp.target = std::max(2.0e4, mults / (op->num_cu * 64.0));
 	if (const char *tm = getenv("BACCHUS_HIP_TILE_MULTS"))
 		pp.target = std::max(1.0e3, atof(tm));
-	pp.bt.assign(nb, 1);
+	pp.bt.assign(2 * nb, 1);
 	Plan plan;
 	if (int rc = make_plan(op, pp, plan))
 		return fail(rc);
@@ -929,6 +1118,20 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	if (precision != BACCHUS_HIP_FP64 &&
 	    (rc = upload((void **)&op->d_xf, nullptr, op->dimension * sizeof(float))))
 		return fail(rc);
+	{
+		std::vector<int32_t> bnp(nb), bnn(nb);
+		for (int64_t b = 0; b < nb; b++) {
+			bnp[b] = (int32_t)op->block_np[b];
+			bnn[b] = (int32_t)op->block_nn[b];
+		}
+		const size_t xs = precision == BACCHUS_HIP_FP64 ? sizeof(double) : sizeof(float);
+		if ((rc = upload(&op->d_xT, nullptr, op->dimension * xs)) ||
+		    (rc = upload((void **)&op->d_yT, nullptr, op->dimension * sizeof(double))) ||
+		    (rc = upload((void **)&op->d_blk_off, op->block_off.data(), nb * sizeof(int64_t))) ||
+		    (rc = upload((void **)&op->d_blk_np, bnp.data(), nb * sizeof(int32_t))) ||
+		    (rc = upload((void **)&op->d_blk_nn, bnn.data(), nb * sizeof(int32_t))))
+			return fail(rc);
+	}
 	*out = op;
 	return 0;
 }
@@ -945,15 +1148,29 @@ int bacchus_hip_operator_matvec(bacchus_hip_operator_t *op, const double *x,
 	if (!accumulate)
 		hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(threads), 0, s, y, n, 0.0);
 	const double s1 = op->scales[0], s2 = op->scales[1], s3 = op->scales[2];
+	const dim3 tgrid((unsigned)op->block_np.size(), kTransposeSplits);
+	if (op->has_transposed)
+		hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(threads), 0, s, op->d_yT, n, 0.0);
 	if (op->precision == BACCHUS_HIP_FP64) {
-		launch<double>(op, (const double *)op->d_v, x, y, s1, s2, s3, s);
+		double *xT = (double *)op->d_xT;
+		if (op->has_transposed)
+			hipLaunchKernelGGL(transpose_blocks<double>, tgrid, dim3(threads), 0, s, x, xT,
+					   op->d_blk_off, op->d_blk_np, op->d_blk_nn, 1);
+		launch<double>(op, (const double *)op->d_v, x, xT, y, s1, s2, s3, s);
 	} else {
+		float *xT = (float *)op->d_xT;
 		hipLaunchKernelGGL(to_float_kernel, dim3(blocks), dim3(threads), 0, s, x, op->d_xf, n);
+		if (op->has_transposed)
+			hipLaunchKernelGGL(transpose_blocks<float>, tgrid, dim3(threads), 0, s, op->d_xf, xT,
+					   op->d_blk_off, op->d_blk_np, op->d_blk_nn, 1);
 		if (op->precision == BACCHUS_HIP_FP32)
-			launch<float>(op, (const float *)op->d_v, op->d_xf, y, s1, s2, s3, s);
+			launch<float>(op, (const float *)op->d_v, op->d_xf, xT, y, s1, s2, s3, s);
 		else
-			launch<uint16_t>(op, (const uint16_t *)op->d_v, op->d_xf, y, s1, s2, s3, s);
+			launch<uint16_t>(op, (const uint16_t *)op->d_v, op->d_xf, xT, y, s1, s2, s3, s);
 	}
+	if (op->has_transposed)
+		hipLaunchKernelGGL(add_transposed_blocks<double>, tgrid, dim3(threads), 0, s, y,
+				   op->d_yT, op->d_blk_off, op->d_blk_np, op->d_blk_nn, 1);
 	BACCHUS_HIP_CHECK(hipGetLastError());
 	return 0;
 }
