@@ -240,6 +240,10 @@ int bacchus_hip_basis_project(const float *U, int64_t n, int64_t D,
 			      const double *V, int64_t B, double *out);
 int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D,
 			      const float *c, double *out, void *stream);
+/*   transform: out[j] = sum_i C[j][i] U_i for j < r (C host FP64 r x n;
+ *              out device FP32 [r][D]) - e.g. an orthonormalized basis */
+int bacchus_hip_basis_transform(const float *U, int64_t n, int64_t D,
+				const double *C, int64_t r, float *out);
 
 /* Learned Lanczos start (no counterpart in the reference): an MLP
  * in -> hidden (x depth, GELU) -> out with BF16 weights on MFMA, FP32

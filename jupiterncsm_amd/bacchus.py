@@ -48,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_batch_create", "bacchus_hip_batch_free", "bacchus_hip_batch_set_scales",
     "bacchus_hip_batch_matvec", "bacchus_hip_batch_autotune", "bacchus_hip_batch_stats",
     "bacchus_hip_batch_lanczos", "bacchus_hip_basis_gram", "bacchus_hip_basis_project",
-    "bacchus_hip_basis_combine",
+    "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
 )
 
 
@@ -166,6 +166,8 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_basis_project.argtypes = [vp, i64, i64, vp, i64, vp]
     lib.bacchus_hip_basis_combine.restype = c_int
     lib.bacchus_hip_basis_combine.argtypes = [vp, i64, i64, vp, vp, vp]
+    lib.bacchus_hip_basis_transform.restype = c_int
+    lib.bacchus_hip_basis_transform.argtypes = [vp, i64, i64, vp, i64, vp]
     _LIB = lib
     return lib
 
@@ -495,6 +497,19 @@ def basis_combine(U, c, out=None, stream=None):
                                                _tensor(c, torch.float32, "c"),
                                                _tensor(out, torch.float64, "out"),
                                                _stream_handle(stream)))
+    return out
+
+
+def basis_transform(U, C):
+    """out[j] = sum_i C[j, i] U[i] on the device (float32 (r, D)); C host."""
+    import torch
+    n, D = U.shape
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    r = C.shape[0]
+    assert C.shape[1] == n
+    out = torch.empty((r, D), dtype=torch.float32, device=U.device)
+    _check(library().bacchus_hip_basis_transform(_tensor(U, torch.float32, "U"), n, D, _ptr(C), r,
+                                                 _tensor(out, torch.float32, "out")))
     return out
 
 

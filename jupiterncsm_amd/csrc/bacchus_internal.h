@@ -90,6 +90,8 @@ constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
 constexpr int64_t kWideRows = 64;
 // rows of a direct tile (N instructions, register sums, atomics into y)
 constexpr int64_t kDirectRows = 32;
+// XCDs of the device: workgroup i of a launch runs on XCD i mod kXcds
+constexpr int kXcds = 8;
 
 // Tiling of one operator's output: the tiles (rows of a block x all
 // neutron states x bt vectors) with their tasks, heaviest first.
@@ -98,6 +100,7 @@ struct PlanParams {
 	std::vector<int> bt;        // vectors per tile, per basis block
 	int64_t tile_elems = kTileMaxElems;  // LDS words per tile (rows*nn*bt)
 	uint64_t task_mults = kTaskMults;    // multiplications x bt per task
+	int xcds = 0;               // > 1: group output blocks per XCD
 };
 struct Plan {
 	std::vector<TileWork> tiles;

@@ -38,8 +38,8 @@ struct BatchOp {
 	float *y;
 	const float *xT;      // per-block transposed copies ([neutron][proton])
 	float *yT;
-	const float *scales;  // [B][4], index 1..3 = force class
-	int32_t B;
+	const float *scales;  // [4][B]: row f = scale of force class f
+	int32_t B;            // vectors, padded to a multiple of 4 (row stride)
 	int32_t precision;
 };
 
@@ -47,12 +47,33 @@ struct BatchItem {
 	int32_t op;
 	int32_t tile;
 	int32_t b0;
-	int32_t lbt;  // log2 of the vectors of the tile
+	int32_t lbt;  // log2 of the vectors of the tile (>= 2)
 };
 
 __device__ __forceinline__ float sign_of(float v, uint32_t m)
 {
 	return __int_as_float(__float_as_int(v) ^ (int)(m & 0x80000000u));
+}
+
+__device__ __forceinline__ float4 ld4(const float *p)
+{
+	return *reinterpret_cast<const float4 *>(p);
+}
+
+__device__ __forceinline__ void fma4(float4 &a, float v, const float4 &x)
+{
+	a.x += v * x.x;
+	a.y += v * x.y;
+	a.z += v * x.z;
+	a.w += v * x.w;
+}
+
+__device__ __forceinline__ void add4_lds(float *t, const float4 &s, const float4 &v)
+{
+	atomicAdd(t + 0, s.x * v.x);
+	atomicAdd(t + 1, s.y * v.y);
+	atomicAdd(t + 2, s.z * v.z);
+	atomicAdd(t + 3, s.w * v.w);
 }
 
 template <typename VT> struct Load;
@@ -66,19 +87,25 @@ template <> struct Load<uint16_t> {
 	}
 };
 
-// One work item: tile w of operator o for vectors [b0, b0 + bt).
-//   NP (matrix_vector_multiplication.c:134-210): lanes = (neutron entry
-//     slot, vector); proton entries of the tile's rows are broadcast with
-//     readlane; sums go to the LDS tile when the proton row changes.
-//   P (:79-132): lanes = (neutron state slot, vector), same broadcast.
-//   N (:20-77): lanes = (entry slot, vector), each entry applied to every
-//     row of the tile.
+// One work item: tile w of operator o for vectors [b0, b0 + bt).  Every
+// lane carries 4 consecutive vectors (one 16-byte load of x per product
+// term, each element value used for 4 FMAs); the lanes of a tile form
+// (entry slot, vector group) pairs: bt / 4 lanes per slot, 64 / (bt / 4)
+// slots per wave.
+//   NP (matrix_vector_multiplication.c:134-210): slots over lane-side
+//     entries; row-side entries of the tile's rows are broadcast with
+//     readlane; sums go to the LDS tile when the row changes.
+//   P (:79-132): slots over columns, same broadcast.
+//   N (:20-77): one output column (a CSR row) at a time, slots striding its
+//     entries, sums for up to 8 tile rows in registers, reduced over the
+//     slots; direct tiles add them to y atomically.
 // N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
 __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 			   float *tile)
 {
 	const int bt = 1 << lbt;
+	const int lq = lbt - 2;       // log2(lanes per slot)
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
 	const bool direct = w.shared == 2;  // N only, sums go straight to y
@@ -90,19 +117,20 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int nwaves = blockDim.x >> 6;
-	const int slot = lane >> lbt;
-	const int G = 64 >> lbt;  // entry slots per pass
-	const int bl = lane & (bt - 1);
+	const int slot = lane >> lq;
+	const int G = 64 >> lq;       // entry slots per pass
+	const int q4 = (lane & ((1 << lq) - 1)) * 4;  // first vector of the lane in the tile
 	const int B = o.B;
-	const int b = b0 + bl;
-	const bool bvalid = b < B;
+	const int b = b0 + q4;
+	const bool bvalid = b < B;    // B is a multiple of 4: all 4 or none
 	const VT *V = (const VT *)o.V;
 	const float *xs = w.space ? o.xT : o.x;
 	for (int32_t ti = w.t_begin + wave; ti < w.t_end; ti += nwaves) {
 		const Task T = o.tasks[ti];
 		const TInstr I = o.ins[T.vi];
-		const float sc = bvalid ? o.scales[b * 4 + I.force] : 0.f;
-		// lane's column of the input block: element e at xb[e * B]
+		const float4 sc = bvalid ? ld4(o.scales + (size_t)I.force * B + b)
+					 : make_float4(0.f, 0.f, 0.f, 0.f);
+		// lane's vectors of the input block: element e at xb + e * B
 		const float *xb = xs + (size_t)I.x_off * B + (bvalid ? b : 0);
 		const uint32_t nn_in = (uint32_t)I.nn_in;
 		if (I.type == 3) {
@@ -116,8 +144,10 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				const uint2 nA = o.ent[kA], nB = o.ent[kB];
 				const uint32_t cA = nA.y & kMask, cB = nB.y & kMask;
 				const uint32_t oA = o.keys[kA], oB = o.keys[kB];
-				const float sA = sign_of(sc, nA.y), sB = sign_of(sc, nB.y);
-				float a0 = 0.f, a1 = 0.f, c0 = 0.f, c1 = 0.f;
+				const float sgA = sign_of(1.f, nA.y), sgB = sign_of(1.f, nB.y);
+				const float4 sA = make_float4(sgA * sc.x, sgA * sc.y, sgA * sc.z, sgA * sc.w);
+				const float4 sB = make_float4(sgB * sc.x, sgB * sc.y, sgB * sc.z, sgB * sc.w);
+				float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, c0 = a0, c1 = a0;
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
@@ -130,18 +160,26 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						if (row != cur) {
 							if (cur >= 0) {
 								const int r = cur - w.po_begin;
-								if (vA)
-									atomicAdd(&tile[((r * nn + (int)oA) << lbt) + bl], sA * (a0 + a1));
-								if (vB)
-									atomicAdd(&tile[((r * nn + (int)oB) << lbt) + bl], sB * (c0 + c1));
+								if (vA) {
+									a0.x += a1.x; a0.y += a1.y; a0.z += a1.z; a0.w += a1.w;
+									add4_lds(&tile[((r * nn + (int)oA) << lbt) + q4], sA, a0);
+								}
+								if (vB) {
+									c0.x += c1.x; c0.y += c1.y; c0.z += c1.z; c0.w += c1.w;
+									add4_lds(&tile[((r * nn + (int)oB) << lbt) + q4], sB, c0);
+								}
 							}
-							a0 = a1 = c0 = c1 = 0.f;
+							a0 = make_float4(0.f, 0.f, 0.f, 0.f);
+							a1 = a0;
+							c0 = a0;
+							c1 = a0;
 							cur = row;
 						}
 						const bool pair = t + 1 < cnt &&
 							(int)__builtin_amdgcn_readlane(prow, t + 1) == row;
 						if (pair) {
-							float va[2], vb[2], xa[2], xc[2];
+							float va[2], vb[2];
+							float4 xa[2], xc[2];
 #pragma unroll
 							for (int u = 0; u < 2; u++) {
 								const uint32_t px = __builtin_amdgcn_readlane(pl.x, t + u);
@@ -150,31 +188,37 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 								const size_t xrow = (size_t)nn_in * px;
 								va[u] = sign_of(Load<VT>::at(vblk, vrow + cA), py);
 								vb[u] = sign_of(Load<VT>::at(vblk, vrow + cB), py);
-								xa[u] = xb[(xrow + nA.x) * B];
-								xc[u] = xb[(xrow + nB.x) * B];
+								xa[u] = ld4(xb + (xrow + nA.x) * B);
+								xc[u] = ld4(xb + (xrow + nB.x) * B);
 							}
-							a0 += va[0] * xa[0];
-							a1 += va[1] * xa[1];
-							c0 += vb[0] * xc[0];
-							c1 += vb[1] * xc[1];
+							fma4(a0, va[0], xa[0]);
+							fma4(a1, va[1], xa[1]);
+							fma4(c0, vb[0], xc[0]);
+							fma4(c1, vb[1], xc[1]);
 							t += 2;
 						} else {
 							const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 							const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
 							const size_t vrow = (size_t)dn * (py & kMask);
 							const size_t xrow = (size_t)nn_in * px;
-							a0 += sign_of(Load<VT>::at(vblk, vrow + cA), py) * xb[(xrow + nA.x) * B];
-							c0 += sign_of(Load<VT>::at(vblk, vrow + cB), py) * xb[(xrow + nB.x) * B];
+							fma4(a0, sign_of(Load<VT>::at(vblk, vrow + cA), py),
+							     ld4(xb + (xrow + nA.x) * B));
+							fma4(c0, sign_of(Load<VT>::at(vblk, vrow + cB), py),
+							     ld4(xb + (xrow + nB.x) * B));
 							t += 1;
 						}
 					}
 				}
 				if (cur >= 0) {
 					const int r = cur - w.po_begin;
-					if (vA)
-						atomicAdd(&tile[((r * nn + (int)oA) << lbt) + bl], sA * (a0 + a1));
-					if (vB)
-						atomicAdd(&tile[((r * nn + (int)oB) << lbt) + bl], sB * (c0 + c1));
+					if (vA) {
+						a0.x += a1.x; a0.y += a1.y; a0.z += a1.z; a0.w += a1.w;
+						add4_lds(&tile[((r * nn + (int)oA) << lbt) + q4], sA, a0);
+					}
+					if (vB) {
+						c0.x += c1.x; c0.y += c1.y; c0.z += c1.z; c0.w += c1.w;
+						add4_lds(&tile[((r * nn + (int)oB) << lbt) + q4], sB, c0);
+					}
 				}
 			}
 		} else if (I.type == 2) {
@@ -182,7 +226,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				const int n = n0 + slot;
 				const bool valid = n < (int)T.e1 && bvalid;
 				const uint32_t nl = n < (int)T.e1 ? (uint32_t)n : T.e0;
-				float a0 = 0.f, a1 = 0.f;
+				float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f);
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
@@ -192,30 +236,25 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					for (uint32_t t = 0; t < cnt; t++) {
 						const int row = (int)__builtin_amdgcn_readlane(prow, t);
 						if (row != cur) {
-							if (cur >= 0 && valid && (a0 + a1) != 0.f)
-								atomicAdd(&tile[(((cur - w.po_begin) * nn + n) << lbt) + bl], sc * (a0 + a1));
-							a0 = a1 = 0.f;
+							if (cur >= 0 && valid)
+								add4_lds(&tile[(((cur - w.po_begin) * nn + n) << lbt) + q4], sc, a0);
+							a0 = make_float4(0.f, 0.f, 0.f, 0.f);
 							cur = row;
 						}
 						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
 						const float e = Load<VT>::at(V, (size_t)I.v_off + (py & kMask));
-						const float xv = xb[((size_t)nn_in * px + nl) * B];
-						if (!(fabsf(e) < 1e-12f)) {
-							if (t & 1)
-								a1 += sign_of(e, py) * xv;
-							else
-								a0 += sign_of(e, py) * xv;
-						}
+						if (!(fabsf(e) < 1e-12f))
+							fma4(a0, sign_of(e, py), ld4(xb + ((size_t)nn_in * px + nl) * B));
 					}
 				}
-				if (cur >= 0 && valid && (a0 + a1) != 0.f)
-					atomicAdd(&tile[(((cur - w.po_begin) * nn + n) << lbt) + bl], sc * (a0 + a1));
+				if (cur >= 0 && valid)
+					add4_lds(&tile[(((cur - w.po_begin) * nn + n) << lbt) + q4], sc, a0);
 			}
 		} else {
 			// CSR rows [jb, je) (output columns): one column at a time,
 			// slots striding its entries, sums for up to 8 tile rows in
-			// registers, then a reduction over the slots of each vector
+			// registers, then a reduction over the slots
 			const float *xrow0 = xb + (size_t)nn_in * w.po_begin * B;
 			for (uint32_t col = T.jb; col < T.je; col++) {
 				const uint32_t kb = o.rowptr[I.n_base + col];
@@ -223,45 +262,52 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				if (kb == ke)
 					continue;
 				for (int r0 = 0; r0 < nrow; r0 += 8) {
-					float acc[8];
+					float4 acc[8];
 #pragma unroll
 					for (int q = 0; q < 8; q++)
-						acc[q] = 0.f;
-					for (uint32_t k = kb + slot; k < ke; k += 4 * G) {
-						uint2 en[4];
-						float v[4];
+						acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+					for (uint32_t k = kb + slot; k < ke; k += 2 * G) {
+						uint2 en[2];
+						float v[2];
 #pragma unroll
-						for (int u = 0; u < 4; u++)
+						for (int u = 0; u < 2; u++)
 							en[u] = k + G * u < ke ? o.ent[k + G * u] : make_uint2(0u, 0u);
 #pragma unroll
-						for (int u = 0; u < 4; u++) {
+						for (int u = 0; u < 2; u++) {
 							const float e = Load<VT>::at(V, (size_t)I.v_off + (en[u].y & kMask));
 							v[u] = (k + G * u < ke && !(fabsf(e) < 1e-12f)) ? sign_of(e, en[u].y)
 													 : 0.f;
 						}
 #pragma unroll
-						for (int u = 0; u < 4; u++) {
+						for (int u = 0; u < 2; u++) {
 							const float *xc = xrow0 + ((size_t)nn_in * r0 + en[u].x) * B;
 #pragma unroll
 							for (int q = 0; q < 8; q++)
 								if (r0 + q < nrow)
-									acc[q] += v[u] * xc[(size_t)nn_in * q * B];
+									fma4(acc[q], v[u], ld4(xc + (size_t)nn_in * q * B));
 						}
 					}
 #pragma unroll
 					for (int q = 0; q < 8; q++) {
 						if (r0 + q >= nrow)
 							break;
-						float v = acc[q];
-						for (int off = 32; off >= bt; off >>= 1)
-							v += __shfl_xor(v, off, 64);
-						if (slot == 0 && bvalid && v != 0.f) {
-							if (direct)
-								atomicAdd(ys + ((size_t)w.y_off +
-										(size_t)(w.po_begin + r0 + q) * nn + col) * B + b,
-									  sc * v);
-							else
-								atomicAdd(&tile[(((r0 + q) * nn + (int)col) << lbt) + bl], sc * v);
+						float4 v = acc[q];
+						for (int off = 32; off >= (1 << lq); off >>= 1) {
+							v.x += __shfl_xor(v.x, off, 64);
+							v.y += __shfl_xor(v.y, off, 64);
+							v.z += __shfl_xor(v.z, off, 64);
+							v.w += __shfl_xor(v.w, off, 64);
+						}
+						if (slot == 0 && bvalid) {
+							if (direct) {
+								float *d = ys + ((size_t)w.y_off +
+										 (size_t)(w.po_begin + r0 + q) * nn + col) * B + b;
+								atomicAdd(d + 0, sc.x * v.x);
+								atomicAdd(d + 1, sc.y * v.y);
+								atomicAdd(d + 2, sc.z * v.z);
+								atomicAdd(d + 3, sc.w * v.w);
+							} else
+								add4_lds(&tile[(((r0 + q) * nn + (int)col) << lbt) + q4], sc, v);
 						}
 					}
 				}
@@ -312,6 +358,28 @@ __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 	}
 }
 
+// dst[r][b] = src[r][b] (b < B), 0 for the padding vectors b in [B, Bp)
+__global__ void pad_cols(const float *__restrict__ src, int64_t D, int B, int Bp,
+			 float *__restrict__ dst)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= D * Bp)
+		return;
+	const int64_t r = i / Bp;
+	const int b = (int)(i - r * Bp);
+	dst[i] = b < B ? src[r * B + b] : 0.f;
+}
+
+__global__ void unpad_cols(const float *__restrict__ src, int64_t D, int B, int Bp,
+			   float *__restrict__ dst)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= D * B)
+		return;
+	const int64_t r = i / B;
+	dst[i] = src[r * Bp + (i - r * B)];
+}
+
 __global__ void reset_queue(int *next)
 {
 	if (threadIdx.x == 0 && blockIdx.x == 0)
@@ -342,7 +410,8 @@ using namespace bacchus;
 
 struct bacchus_hip_batch {
 	std::vector<bacchus_hip_operator_t *> ops;
-	std::vector<int32_t> B;
+	std::vector<int32_t> B;   // vectors per operator
+	std::vector<int32_t> Bp;  // padded to a multiple of 4 (row stride)
 	bacchus_hip_batch_tuning_t tuning{};
 	std::vector<float *> d_scales;
 	std::vector<TileWork *> d_tiles;
@@ -354,7 +423,7 @@ struct bacchus_hip_batch {
 	int64_t num_items = 0;
 	size_t lds_bytes = 0;
 	int grid = 0;
-	// internal vector buffers (autotuning, Lanczos)
+	// internal [D][Bp] vector buffers (padding, autotuning, Lanczos)
 	std::vector<float *> d_x, d_y;
 	// per-operator transposed copies of x and y (operators with
 	// transposed-space instructions)
@@ -413,7 +482,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 	const int num_cu = bt->ops[0]->num_cu;
 	double work = 0;
 	for (int i = 0; i < nops; i++)
-		work += bt->ops[i]->mults * bt->B[i];
+		work += bt->ops[i]->mults * bt->Bp[i];
 	const double target = std::max(2.0e4, work / (double(num_cu) * tiles_per_cu));
 	struct Cand {
 		double cost;
@@ -424,8 +493,8 @@ int build_plan(bacchus_hip_batch_t *bt)
 	for (int i = 0; i < nops; i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
 		const int64_t nb = (int64_t)op->block_nn.size();
-		const int vec = std::min(max_vec, pow2_floor(std::max(1, bt->B[i])) *
-						    (pow2_floor(bt->B[i]) < bt->B[i] ? 2 : 1));
+		const int Bp = bt->Bp[i];
+		const int vec = std::max(4, std::min(max_vec, pow2_floor(Bp) * (pow2_floor(Bp) < Bp ? 2 : 1)));
 		PlanParams pp;
 		pp.target = target;
 		// row widths: neutron states (normal space), proton states
@@ -433,12 +502,12 @@ int build_plan(bacchus_hip_batch_t *bt)
 		int64_t widest = 1;
 		for (int64_t b = 0; b < nb; b++)
 			widest = std::max({widest, op->block_nn[b], op->block_np[b]});
-		pp.tile_elems = std::max<int64_t>(words, widest);
+		pp.tile_elems = std::max<int64_t>(words, 4 * widest);
 		pp.bt.resize(2 * nb);
 		for (int64_t slot = 0; slot < 2 * nb; slot++) {
 			const int64_t width = slot < nb ? op->block_nn[slot] : op->block_np[slot - nb];
 			int v = vec;
-			while (v > 1 && width * v > pp.tile_elems)
+			while (v > 4 && width * v > pp.tile_elems)
 				v >>= 1;
 			pp.bt[slot] = v;
 		}
@@ -465,7 +534,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 			int lbt = 0;
 			while ((1 << lbt) < v)
 				lbt++;
-			for (int b0 = 0; b0 < bt->B[i]; b0 += v)
+			for (int b0 = 0; b0 < Bp; b0 += v)
 				all.push_back(Cand{plan.cost[t], BatchItem{i, (int32_t)t, b0, lbt}});
 		}
 	}
@@ -491,14 +560,14 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 {
 	for (size_t i = 0; i < bt->ops.size(); i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
-		const int64_t n = op->dimension * bt->B[i];
+		const int64_t n = op->dimension * bt->Bp[i];
 		hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->h_ops[i].y, n, 0.f);
 		if (op->has_transposed) {
 			hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_yT[i], n, 0.f);
 			hipLaunchKernelGGL(transpose_blocks<float>,
 					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
 					   bt->h_ops[i].x, bt->d_xT[i], op->d_blk_off, op->d_blk_np,
-					   op->d_blk_nn, bt->B[i]);
+					   op->d_blk_nn, bt->Bp[i]);
 		}
 	}
 	if (bt->num_items) {
@@ -512,7 +581,7 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 			hipLaunchKernelGGL(add_transposed_blocks<float>,
 					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
 					   bt->h_ops[i].y, bt->d_yT[i], op->d_blk_off, op->d_blk_np,
-					   op->d_blk_nn, bt->B[i]);
+					   op->d_blk_nn, bt->Bp[i]);
 	}
 	BACCHUS_HIP_CHECK(hipGetLastError());
 	return 0;
@@ -521,7 +590,7 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 int ensure_buffers(bacchus_hip_batch_t *bt)
 {
 	for (size_t i = 0; i < bt->ops.size(); i++) {
-		const size_t n = (size_t)bt->ops[i]->dimension * bt->B[i];
+		const size_t n = (size_t)bt->ops[i]->dimension * bt->Bp[i];
 		if (!bt->d_x[i]) {
 			BACCHUS_HIP_CHECK(hipMalloc(&bt->d_x[i], n * sizeof(float)));
 			BACCHUS_HIP_CHECK(hipMemset(bt->d_x[i], 0, n * sizeof(float)));
@@ -580,6 +649,8 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		bt->tuning = *tuning;
 	bt->ops.assign(ops, ops + num_ops);
 	bt->B.assign(num_vectors, num_vectors + num_ops);
+	for (int i = 0; i < num_ops; i++)
+		bt->Bp.push_back((num_vectors[i] + 3) / 4 * 4);
 	bt->d_x.assign(num_ops, nullptr);
 	bt->d_y.assign(num_ops, nullptr);
 	bt->d_xT.assign(num_ops, nullptr);
@@ -591,8 +662,13 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 	};
 	for (int i = 0; i < num_ops; i++) {
 		const bacchus_hip_operator_t *op = ops[i];
-		const int B = num_vectors[i];
-		std::vector<float> sc((size_t)B * 4, 1.0f);
+		const int B = num_vectors[i], Bp = bt->Bp[i];
+		// [4][Bp]: scale of force class f for vector b at f * Bp + b;
+		// padding vectors scaled by 0
+		std::vector<float> sc((size_t)Bp * 4, 0.0f);
+		for (int f = 1; f < 4; f++)
+			for (int b = 0; b < B; b++)
+				sc[(size_t)f * Bp + b] = 1.0f;
 		float *ds = nullptr;
 		if (hipMalloc(&ds, sc.size() * sizeof(float)) != hipSuccess)
 			return fail(BACCHUS_FAIL("hipMalloc of the scales failed"));
@@ -600,7 +676,7 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		if (hipMemcpy(ds, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
 			return fail(BACCHUS_FAIL("upload of the scales failed"));
 		if (op->has_transposed) {
-			const size_t n = (size_t)op->dimension * B * sizeof(float);
+			const size_t n = (size_t)op->dimension * Bp * sizeof(float);
 			if (hipMalloc(&bt->d_xT[i], n) != hipSuccess ||
 			    hipMalloc(&bt->d_yT[i], n) != hipSuccess)
 				return fail(BACCHUS_FAIL("hipMalloc of the transposed vectors failed"));
@@ -615,7 +691,7 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		o.keys = op->d_keys;
 		o.V = op->d_v;
 		o.scales = ds;
-		o.B = B;
+		o.B = Bp;
 		o.precision = op->precision;
 		const double sv = op->precision == BACCHUS_HIP_FP32 ? 4 : 2;
 		bt->bytes += sv * double(op->element_count) + 12.0 * double(op->list_entry_count) +
@@ -626,6 +702,8 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 	    hipMalloc(&bt->d_next, sizeof(int)) != hipSuccess)
 		return fail(BACCHUS_FAIL("hipMalloc of the descriptors failed"));
 	if (int rc = build_plan(bt))
+		return fail(rc);
+	if (int rc = use_internal(bt))
 		return fail(rc);
 	*out = bt;
 	return 0;
@@ -639,11 +717,11 @@ int bacchus_hip_batch_set_scales(bacchus_hip_batch_t *bt, int32_t i, const doubl
 		return BACCHUS_FAIL("NULL argument");
 	if (i < 0 || i >= (int32_t)bt->ops.size())
 		return BACCHUS_FAIL("operator index %d out of range", i);
-	const int B = bt->B[i];
-	std::vector<float> sc((size_t)B * 4, 0.0f);
+	const int B = bt->B[i], Bp = bt->Bp[i];
+	std::vector<float> sc((size_t)Bp * 4, 0.0f);
 	for (int b = 0; b < B; b++)
 		for (int f = 0; f < 3; f++)
-			sc[(size_t)b * 4 + 1 + f] = (float)scales[(size_t)b * 3 + f];
+			sc[(size_t)(f + 1) * Bp + b] = (float)scales[(size_t)b * 3 + f];
 	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_scales[i], sc.data(), sc.size() * sizeof(float),
 				    hipMemcpyHostToDevice));
 	return 0;
@@ -654,22 +732,45 @@ int bacchus_hip_batch_matvec(bacchus_hip_batch_t *bt, const float *const *x, flo
 {
 	if (!bt || !x || !y)
 		return BACCHUS_FAIL("NULL argument");
+	hipStream_t s = (hipStream_t)stream;
 	bool changed = false;
+	std::vector<char> direct(bt->ops.size());
 	for (size_t i = 0; i < bt->ops.size(); i++) {
 		if (!x[i] || !y[i])
 			return BACCHUS_FAIL("NULL vector for operator %zu", i);
-		if (bt->h_ops[i].x != x[i] || bt->h_ops[i].y != y[i]) {
-			bt->h_ops[i].x = x[i];
-			bt->h_ops[i].y = y[i];
+		// caller arrays are used in place when their rows are whole
+		// 16-byte groups; otherwise they go through the padded buffers
+		direct[i] = bt->B[i] == bt->Bp[i] && ((uintptr_t)x[i] & 15) == 0 &&
+			    ((uintptr_t)y[i] & 15) == 0;
+		const float *xi = direct[i] ? x[i] : bt->d_x[i];
+		float *yi = direct[i] ? y[i] : bt->d_y[i];
+		if (bt->h_ops[i].x != xi || bt->h_ops[i].y != yi) {
+			bt->h_ops[i].x = xi;
+			bt->h_ops[i].y = yi;
 			changed = true;
 		}
 	}
 	if (changed) {
-		BACCHUS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+		BACCHUS_HIP_CHECK(hipStreamSynchronize(s));
 		if (int rc = upload_ops(bt))
 			return rc;
 	}
-	return launch_batch(bt, (hipStream_t)stream);
+	for (size_t i = 0; i < bt->ops.size(); i++)
+		if (!direct[i]) {
+			const int64_t D = bt->ops[i]->dimension;
+			hipLaunchKernelGGL(pad_cols, dim3(blocks_of(D * bt->Bp[i])), dim3(256), 0, s, x[i], D,
+					   bt->B[i], bt->Bp[i], bt->d_x[i]);
+		}
+	if (int rc = launch_batch(bt, s))
+		return rc;
+	for (size_t i = 0; i < bt->ops.size(); i++)
+		if (!direct[i]) {
+			const int64_t D = bt->ops[i]->dimension;
+			hipLaunchKernelGGL(unpad_cols, dim3(blocks_of(D * bt->B[i])), dim3(256), 0, s,
+					   bt->d_y[i], D, bt->B[i], bt->Bp[i], y[i]);
+		}
+	BACCHUS_HIP_CHECK(hipGetLastError());
+	return 0;
 }
 
 int bacchus_hip_batch_stats(const bacchus_hip_batch_t *bt, double *bytes, double *flops,
@@ -780,19 +881,27 @@ struct LState {
 	int64_t *k = nullptr;     // iteration
 };
 
-__global__ void to_float_cols(const double *__restrict__ Q, const int64_t *kp, int64_t n,
-			      float *__restrict__ xf)
+// xf[r][b] = Q_k[r][b] into the padded [D][Bp] matvec input
+__global__ void to_float_cols(const double *__restrict__ Q, const int64_t *kp, int64_t D, int B,
+			      int Bp, float *__restrict__ xf)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		xf[i] = (float)Q[*kp * n + i];
+	if (i >= D * Bp)
+		return;
+	const int64_t r = i / Bp;
+	const int b = (int)(i - r * Bp);
+	xf[i] = b < B ? (float)Q[*kp * D * B + r * B + b] : 0.f;
 }
 
-__global__ void to_double_cols(const float *__restrict__ yf, int64_t n, double *__restrict__ w)
+// w[r][b] = yf[r][b] from the padded matvec output
+__global__ void to_double_cols(const float *__restrict__ yf, int64_t D, int B, int Bp,
+			       double *__restrict__ w)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		w[i] = (double)yf[i];
+	if (i >= D * B)
+		return;
+	const int64_t r = i / B;
+	w[i] = (double)yf[r * Bp + (i - r * B)];
 }
 
 // partial[j][c][b] = sum over rows of chunk c of U_j[r][b] * w[r][b],
@@ -990,17 +1099,17 @@ struct BatchLanczos {
 	{
 		for (size_t i = 0; i < st.size(); i++) {
 			LState &l = st[i];
-			const int64_t n = l.D * l.B;
-			hipLaunchKernelGGL(to_float_cols, dim3(blocks_of(n)), dim3(256), 0, s, l.Q, l.k, n,
-					   bt->d_x[i]);
+			const int Bp = bt->Bp[i];
+			hipLaunchKernelGGL(to_float_cols, dim3(blocks_of(l.D * Bp)), dim3(256), 0, s, l.Q, l.k,
+					   l.D, l.B, Bp, bt->d_x[i]);
 		}
 		if (int rc = launch_batch(bt, s))
 			return rc;
 		for (size_t i = 0; i < st.size(); i++) {
 			LState &l = st[i];
 			const int64_t n = l.D * l.B;
-			hipLaunchKernelGGL(to_double_cols, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_y[i], n,
-					   l.w);
+			hipLaunchKernelGGL(to_double_cols, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_y[i], l.D,
+					   l.B, bt->Bp[i], l.w);
 			project(l, 0);
 			project(l, 1);
 			norm_sq(l, l.w);

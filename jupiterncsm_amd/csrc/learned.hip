@@ -85,6 +85,20 @@ __global__ void combine_basis(const float *__restrict__ U, int n, int64_t D,
 	out[r] = v;
 }
 
+// out[j][r] = sum_i C[j][i] U[i][r] (FP64 accumulation), grid.y = j
+__global__ void transform_basis(const float *__restrict__ U, int n, int64_t D,
+				const double *__restrict__ C, float *__restrict__ out)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= D)
+		return;
+	const double *c = C + (int64_t)blockIdx.y * n;
+	double v = 0;
+	for (int i = 0; i < n; i++)
+		v += c[i] * (double)U[(int64_t)i * D + r];
+	out[(int64_t)blockIdx.y * D + r] = (float)v;
+}
+
 // one block: ss = |out|^2 in a fixed order, then out /= sqrt(ss)
 __global__ __launch_bounds__(256) void normalize_vector(double *out, int64_t D)
 {
@@ -169,6 +183,25 @@ int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D, const float 
 			   D, c, out);
 	hipLaunchKernelGGL(normalize_vector, dim3(1), dim3(256), 0, s, out, D);
 	BACCHUS_HIP_CHECK(hipGetLastError());
+	return 0;
+}
+
+int bacchus_hip_basis_transform(const float *U, int64_t n, int64_t D, const double *C, int64_t r,
+				float *out)
+{
+	if (!U || !C || !out || n <= 0 || D <= 0 || r <= 0 || r > 65535)
+		return BACCHUS_FAIL("bad arguments");
+	double *dc = nullptr;
+	BACCHUS_HIP_CHECK(hipMalloc(&dc, (size_t)r * n * sizeof(double)));
+	hipError_t e = hipMemcpy(dc, C, (size_t)r * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess) {
+		hipLaunchKernelGGL(transform_basis, dim3((unsigned)((D + 255) / 256), (unsigned)r), dim3(256),
+				   0, nullptr, U, (int)n, D, dc, out);
+		e = hipDeviceSynchronize();
+	}
+	(void)hipFree(dc);
+	if (e != hipSuccess)
+		return BACCHUS_FAIL("transform: %s", hipGetErrorString(e));
 	return 0;
 }
 

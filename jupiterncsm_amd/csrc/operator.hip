@@ -559,6 +559,58 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	out.tiles.swap(tiles);
 	out.cost.swap(costs);
 	out.bt = bts;
+	if (pp.xcds > 1 && !out.tiles.empty()) {
+		// XCD-aware order: workgroup i runs on XCD i mod xcds, so every
+		// output block (its element sub-blocks are walked by all its tiles
+		// in the same instruction order) goes to one XCD and its tiles are
+		// dispatched together there, sharing that XCD's L2.  Blocks are
+		// balanced over the XCDs by cost; holes are padded with empty
+		// tiles.
+		std::map<std::pair<int, int64_t>, std::vector<size_t>> groups;
+		for (size_t k = 0; k < out.tiles.size(); k++)
+			groups[{out.tiles[k].space, out.tiles[k].y_off}].push_back(k);
+		std::vector<std::pair<double, std::vector<size_t>>> gl;
+		for (auto &g : groups) {
+			double c = 0;
+			for (size_t k : g.second)
+				c += out.cost[k];
+			gl.push_back({c, g.second});
+		}
+		std::stable_sort(gl.begin(), gl.end(),
+				 [](const auto &a, const auto &b) { return a.first > b.first; });
+		const int X = pp.xcds;
+		std::vector<double> load(X, 0.0);
+		std::vector<std::vector<size_t>> bins(X);
+		for (auto &g : gl) {
+			const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+			load[k] += g.first;
+			bins[k].insert(bins[k].end(), g.second.begin(), g.second.end());
+		}
+		size_t len = 0;
+		for (auto &b : bins)
+			len = std::max(len, b.size());
+		TileWork empty{};
+		empty.space = 0;
+		std::vector<TileWork> t2;
+		std::vector<double> c2;
+		std::vector<int> b2;
+		for (size_t r = 0; r < len; r++)
+			for (int k = 0; k < X; k++) {
+				if (r < bins[k].size()) {
+					const size_t i = bins[k][r];
+					t2.push_back(out.tiles[i]);
+					c2.push_back(out.cost[i]);
+					b2.push_back(out.bt[i]);
+				} else {
+					t2.push_back(empty);
+					c2.push_back(0.0);
+					b2.push_back(1);
+				}
+			}
+		out.tiles.swap(t2);
+		out.cost.swap(c2);
+		out.bt.swap(b2);
+	}
 	for (size_t ti = 0; ti < out.tiles.size(); ti++) {
 		TileWork &t = out.tiles[ti];
 		if (t.shared == 2) {
@@ -569,7 +621,7 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 			t.t_end = (int32_t)out.tasks.size();
 			continue;
 		}
-		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / bts[ti]);
+		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / out.bt[ti]);
 		t.t_begin = (int32_t)out.tasks.size();
 		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
 			const TInstr &I = tinstr[v];
@@ -1046,6 +1098,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	if (const char *tm = getenv("BACCHUS_HIP_TILE_MULTS"))
 		pp.target = std::max(1.0e3, atof(tm));
 	pp.bt.assign(2 * nb, 1);
+	pp.xcds = kXcds;
+	if (const char *xe = getenv("BACCHUS_HIP_XCDS"))
+		pp.xcds = atoi(xe);
 	Plan plan;
 	if (int rc = make_plan(op, pp, plan))
 		return fail(rc);

@@ -138,21 +138,37 @@ class LearnedStart:
         return v
 
 
-def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
-        ridge=1e-8, seed=1, target="ground", log=None):
-    """Stored basis = the first basis_size targets; MLP trained on the rest
-    (all points) to the least-squares coefficients over that basis."""
-    import torch
-    T = solves.targets
-    U = T[:basis_size].contiguous()
+def orthonormal_basis(U, threshold=1e-12):
+    """Orthonormal combinations of the stored vectors U (n, D): from the
+    Gram matrix G = V diag(w) V^T, rows w_j^-1/2 V_j^T U for the w_j above
+    threshold * max(w) (directions below it are noise in FP32 storage);
+    padded with zero rows to n rows.  Returns (Q (n, D), rank)."""
     n = U.shape[0]
     G = bacchus.basis_gram(U)
-    G += ridge * np.trace(G) / n * np.eye(n)
-    # coefficients of every target: (U U^T) z = U t, in chunks of 64 vectors
+    w, V = np.linalg.eigh(0.5 * (G + G.T))
+    keep = w > threshold * w.max()
+    C = np.zeros((n, n))
+    C[:keep.sum()] = (V[:, keep] / np.sqrt(w[keep])).T[::-1]
+    return bacchus.basis_transform(U, C), int(keep.sum())
+
+
+def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
+        seed=1, target="ground", log=None):
+    """Stored basis: the orthonormalized span of the first basis_size
+    targets (a fixed set of basis_size FP32 vectors, zero rows past the
+    numerical rank).  The MLP is trained on all points to the coordinates
+    of their targets in that basis: bounded, smooth in (c1, c2), and an
+    error in them is the same error in the start vector."""
+    import torch
+    T = solves.targets
+    U, rank = orthonormal_basis(T[:basis_size].contiguous())
+    n = U.shape[0]
+    if log:
+        log(f"basis rank {rank} of {n}")
     Z = []
     for s in range(0, T.shape[0], 64):
         V = T[s:s + 64].to(torch.float64).T.contiguous()       # (D, b)
-        Z.append(np.linalg.solve(G, bacchus.basis_project(U, V)).T)
+        Z.append(bacchus.basis_project(U, V).T)
     Z = np.concatenate(Z, 0)                                     # (P, n)
     mean = Z.mean(0)
     scale = float(np.abs(Z - mean).max()) or 1.0
@@ -163,7 +179,8 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
     loss = None
     for k in range(steps):
         want = (k + 1) % 500 == 0 or k + 1 == steps
-        step_lr = lr * (0.1 if k >= 0.75 * steps else 1.0)
+        # cosine decay to 1% of lr
+        step_lr = lr * (0.01 + 0.99 * 0.5 * (1.0 + np.cos(np.pi * k / steps)))
         loss = mlp.train_step(xd, td, lr=step_lr, want_loss=want) if want else \
             mlp.train_step(xd, td, lr=step_lr)
         if want and log:
