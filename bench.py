@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--train-iterations", type=int, default=80)
     ap.add_argument("--no-learned", action="store_true",
                     help="fast path from a random start instead of the learned one")
+    ap.add_argument("--start-mix", type=float, default=1e-4,
+                    help="relative size of the random component mixed into the learned start")
     ap.add_argument("--batched", action="store_true", help="batched 64-Hamiltonian sweep")
     ap.add_argument("--learned", action="store_true", help="learned-start measurement")
     ap.add_argument("--instances", type=int, default=64)
@@ -180,7 +182,7 @@ def main():
         kw = dict(num_eigenvalues=args.eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
                   convergence=bacchus.CONVERGE_TARGETS, tolerance=args.tolerance,
                   target_eigenvalues=targets, check_every=1)
-        sv = model.start_vector(1.0, 1.0, mix=0.01) if learned else None
+        sv = model.start_vector(1.0, 1.0, mix=args.start_mix) if learned else None
         if sv is not None:
             r = bacchus.diagonalize(op, args.iterations, start=bacchus.START_GIVEN,
                                     start_vector=sv, **kw)
