@@ -41,7 +41,8 @@ struct TInstr {
 	int32_t nn_in;     // neutron states of the input block
 	int16_t type;      // 1 N, 2 P, 3 NP
 	int16_t force;     // 1, 2, 3 body
-	int32_t pad;
+	int32_t expanded;  // NP: 1 = element rows expanded into lane-list
+			   // order (column = entry position, row stride n_len)
 };
 
 // One output tile: rows [po_begin, po_end) of a basis block (all columns)
@@ -88,6 +89,8 @@ constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
 // P instructions of blocks with fewer neutron states than this run in the
 // transposed space (as N instructions there)
 constexpr int64_t kWideRows = 64;
+// lane-side NP entries held by one lane of the single-vector kernel
+constexpr int kLaneEntries = 4;
 // rows of a direct tile (N instructions, register sums, atomics into y)
 constexpr int64_t kDirectRows = 32;
 // XCDs of the device: workgroup i of a launch runs on XCD i mod kXcds

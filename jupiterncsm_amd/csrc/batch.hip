@@ -142,7 +142,10 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				const uint32_t kA = I.n_base + (eA < T.e1 ? eA : T.e0);
 				const uint32_t kB = I.n_base + (eB < T.e1 ? eB : T.e0);
 				const uint2 nA = o.ent[kA], nB = o.ent[kB];
-				const uint32_t cA = nA.y & kMask, cB = nB.y & kMask;
+				// element column: compact index, or lane-list position for
+				// expanded element rows
+				const uint32_t cA = I.expanded ? kA - I.n_base : nA.y & kMask;
+				const uint32_t cB = I.expanded ? kB - I.n_base : nB.y & kMask;
 				const uint32_t oA = o.keys[kA], oB = o.keys[kB];
 				const float sgA = sign_of(1.f, nA.y), sgB = sign_of(1.f, nB.y);
 				const float4 sA = make_float4(sgA * sc.x, sgA * sc.y, sgA * sc.z, sgA * sc.w);

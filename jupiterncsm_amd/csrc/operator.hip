@@ -133,83 +133,77 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 		const TInstr I = ins[T.vi];
 		const acc_t sc = acc_t(I.force == 1 ? s1 : (I.force == 2 ? s2 : s3));
 		if (I.type == 3) {
-			// proton entries [jb, je) of the tile's rows (contiguous in the
-			// copy ordered by row; keys[] gives each entry's row): lanes walk
-			// neighbouring neutron entries, i.e. neighbouring columns of the
-			// dense sub-block, two chunks of 64 at a time.  Proton entries
+			// row-side entries [jb, je) of the tile's rows (contiguous in
+			// the copy ordered by row; keys[] gives each entry's row):
+			// every lane holds kLaneEntries neighbouring lane-side entries
+			// (chunks of 64), so each broadcast row-side entry puts
+			// 2 kLaneEntries independent loads in flight.  Row-side entries
 			// arrive with one coalesced load per 64 and are broadcast with
-			// readlane; a lane's partial sum goes to the tile whenever the
+			// readlane; a lane's partial sums go to the tile whenever the
 			// (uniform) row changes.
 			const VT *vblk = V + I.v_off;
 			const auto *xblk = x + I.x_off;
 			const uint32_t dn = (uint32_t)I.dn, nn_in = (uint32_t)I.nn_in;
-			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 128) {
-				const uint32_t eA = e0 + lane, eB = e0 + 64 + lane;
-				const bool vA = eA < T.e1, vB = eB < T.e1;
-				const uint32_t kA = I.n_base + (vA ? eA : T.e0);
-				const uint32_t kB = I.n_base + (vB ? eB : T.e0);
-				const uint2 nA = ent[kA], nB = ent[kB];
-				const uint32_t cA = nA.y & kIndexMask, cB = nB.y & kIndexMask;
-				const uint32_t oA = keys[kA], oB = keys[kB];
-				const acc_t sA = apply_sign(sc, nA.y), sB = apply_sign(sc, nB.y);
-				acc_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 64 * kLaneEntries) {
+				bool vl[kLaneEntries];
+				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
+				acc_t sg[kLaneEntries], acc[kLaneEntries];
+#pragma unroll
+				for (int c = 0; c < kLaneEntries; c++) {
+					const uint32_t e = e0 + 64 * c + lane;
+					vl[c] = e < T.e1;
+					const uint32_t k = I.n_base + (vl[c] ? e : T.e0);
+					const uint2 en = ent[k];
+					// element column: the lane entry's compact index, or
+					// its position in the lane list for expanded rows
+					col[c] = I.expanded ? k - I.n_base : en.y & kIndexMask;
+					xin[c] = en.x;
+					outc[c] = keys[k];
+					sg[c] = apply_sign(sc, en.y);
+					acc[c] = 0;
+				}
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
 					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
 					const uint2 pl = ent[jl];
 					const uint32_t prow = keys[jl];
-					uint32_t t = 0;
-					while (t < cnt) {
+					for (uint32_t t = 0; t < cnt; t++) {
 						const int row = (int)__builtin_amdgcn_readlane(prow, t);
 						if (row != cur) {
 							if (cur >= 0) {
 								const int r = cur - w.po_begin;
-								if (vA)
-									atomicAdd(&tile[r * nn + oA], (double)(sA * (a0 + a1)));
-								if (vB)
-									atomicAdd(&tile[r * nn + oB], (double)(sB * (b0 + b1)));
+#pragma unroll
+								for (int c = 0; c < kLaneEntries; c++)
+									if (vl[c])
+										atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 							}
-							a0 = a1 = b0 = b1 = 0;
+#pragma unroll
+							for (int c = 0; c < kLaneEntries; c++)
+								acc[c] = 0;
 							cur = row;
 						}
-						const bool pair = t + 1 < cnt &&
-							(int)__builtin_amdgcn_readlane(prow, t + 1) == row;
-						if (pair) {
-							acc_t va[2], vb[2], xa[2], xb[2];
+						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+						const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+						const auto *xcol = xblk + (size_t)(nn_in * px);
+						acc_t va[kLaneEntries], xa[kLaneEntries];
 #pragma unroll
-							for (int u = 0; u < 2; u++) {
-								const uint32_t px = __builtin_amdgcn_readlane(pl.x, t + u);
-								const uint32_t py = __builtin_amdgcn_readlane(pl.y, t + u);
-								const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
-								const auto *xcol = xblk + (size_t)(nn_in * px);
-								va[u] = apply_sign(Elem<VT>::load(vrow, cA), py);
-								vb[u] = apply_sign(Elem<VT>::load(vrow, cB), py);
-								xa[u] = acc_t(xcol[nA.x]);
-								xb[u] = acc_t(xcol[nB.x]);
-							}
-							a0 += va[0] * xa[0];
-							a1 += va[1] * xa[1];
-							b0 += vb[0] * xb[0];
-							b1 += vb[1] * xb[1];
-							t += 2;
-						} else {
-							const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
-							const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-							const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
-							const auto *xcol = xblk + (size_t)(nn_in * px);
-							a0 += apply_sign(Elem<VT>::load(vrow, cA), py) * acc_t(xcol[nA.x]);
-							b0 += apply_sign(Elem<VT>::load(vrow, cB), py) * acc_t(xcol[nB.x]);
-							t += 1;
+						for (int c = 0; c < kLaneEntries; c++) {
+							va[c] = Elem<VT>::load(vrow, col[c]);
+							xa[c] = acc_t(xcol[xin[c]]);
 						}
+#pragma unroll
+						for (int c = 0; c < kLaneEntries; c++)
+							acc[c] += apply_sign(va[c], py) * xa[c];
 					}
 				}
 				if (cur >= 0) {
 					const int r = cur - w.po_begin;
-					if (vA)
-						atomicAdd(&tile[r * nn + oA], (double)(sA * (a0 + a1)));
-					if (vB)
-						atomicAdd(&tile[r * nn + oB], (double)(sB * (b0 + b1)));
+#pragma unroll
+					for (int c = 0; c < kLaneEntries; c++)
+						if (vl[c])
+							atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 				}
 			}
 		} else if (I.type == 2) {
@@ -646,7 +640,7 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 			if (je <= jb)
 				continue;
 			const uint32_t span = I.type == 3 ? I.n_len : (uint32_t)t.nn_out;
-			const uint32_t unit = I.type == 3 ? 128u : 64u;
+			const uint32_t unit = I.type == 3 ? 64u * kLaneEntries : 64u;
 			const uint32_t e_step = std::min<uint32_t>((span + unit - 1) / unit * unit, 1024u);
 			const uint32_t j_step = std::max<uint32_t>(64u, (uint32_t)(task_mults / e_step));
 			for (uint32_t jj = jb; jj < je; jj += j_step)
@@ -897,11 +891,28 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		else if (I.type == 2)
 			swapped[i] = op->block_nn[I.block_out] < kWideRows;
 	}
+	// lane-side lists whose elements are used at most about twice (the
+	// 3-body NP lists): their instructions store the element rows expanded
+	// into lane-entry order, so lane loads of elements are contiguous and
+	// the lane list can be ordered by input index (clustered x loads)
+	std::vector<char> expanded(nl, 0);
+	for (int64_t i = 0; i < ni; i++) {
+		const bacchus_hip_instruction_t &I = p->instructions[i];
+		if (I.type != 3)
+			continue;
+		const int32_t l = swapped[i] ? I.proton_list : I.neutron_list;
+		expanded[l] = list_len(l) <= 2 * (int64_t)mvals[l].size();
+	}
+	auto lane_list = [&](const bacchus_hip_instruction_t &I, int64_t i) -> int32_t {
+		return swapped[i] ? I.proton_list : I.neutron_list;
+	};
 	std::map<std::tuple<int32_t, int32_t, int32_t>, int64_t> sub_of;
 	std::vector<int64_t> sub_off(ni, -1);
 	std::vector<double> W;
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
+		if (I.type == 3 && expanded[lane_list(I, i)])
+			continue;  // built per direction below
 		const auto key = std::make_tuple(I.matrix_block, I.type != 2 ? I.neutron_list : -1,
 						 I.type != 1 ? I.proton_list : -1);
 		auto it = sub_of.find(key);
@@ -935,7 +946,6 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		sub_of[key] = off;
 		sub_off[i] = off;
 	}
-	op->compact_elements = (int64_t)W.size();
 
 	// --- CSR copies of every used list, by output and by input index,
 	// each row ordered by element index; keys[k] = row of entry k ---
@@ -1001,9 +1011,15 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
 			std::vector<uint32_t> perm(b - a);
 			std::iota(perm.begin(), perm.end(), a);
-			std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
-				return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
-			});
+			if (expanded[l])
+				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+					return std::make_pair(entries[u].x, entries[u].y & 0x7FFFFFFFu) <
+					       std::make_pair(entries[w].x, entries[w].y & 0x7FFFFFFFu);
+				});
+			else
+				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+					return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
+				});
 			(dir == 0 ? elem_by_out : elem_by_in)[l] = (int64_t)entries.size();
 			for (uint32_t k : perm) {
 				const uint2 e = entries[k];
@@ -1015,6 +1031,37 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		if (entries.size() > 0xFFFFFFFFull)
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
 	}
+
+	// expanded element rows of an NP instruction in one direction: row =
+	// compact index of the row-side list, column = position in the lane
+	// copy of that direction
+	std::map<std::tuple<int32_t, int32_t, int32_t, int>, int64_t> exp_of;
+	auto expanded_rows = [&](const bacchus_hip_instruction_t &I, int64_t i, int dir) -> int64_t {
+		const auto key = std::make_tuple(I.matrix_block, I.neutron_list, I.proton_list, dir);
+		auto it = exp_of.find(key);
+		if (it != exp_of.end())
+			return it->second;
+		const int32_t l = lane_list(I, i);
+		const int32_t rl = swapped[i] ? I.neutron_list : I.proton_list;
+		const int64_t base = (dir == 0 ? elem_by_out : elem_by_in)[l];
+		const int64_t len = list_len(l);
+		const auto &rows = mvals[rl];
+		const auto &cols = mvals[l];
+		const double *Vb = p->matrix_elements + p->matrix_offsets[I.matrix_block];
+		const int64_t Dn = vdn[I.matrix_block];
+		const int64_t off = (int64_t)W.size();
+		W.resize(off + (int64_t)rows.size() * len);
+		double *dst = W.data() + off;
+		for (size_t r = 0; r < rows.size(); r++)
+			for (int64_t k = 0; k < len; k++) {
+				const uint32_t m = cols[entries[base + k].y & 0x7FFFFFFFu];
+				// element index m_n + Dn m_p
+				dst[r * len + k] = swapped[i] ? Vb[rows[r] + Dn * (int64_t)m]
+							      : Vb[m + Dn * (int64_t)rows[r]];
+			}
+		exp_of[key] = off;
+		return off;
+	};
 
 	// --- virtual instructions (each direction) grouped by output block;
 	// slots [0, nb) are the normal space, [nb, 2 nb) the transposed one ---
@@ -1037,6 +1084,11 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			t.v_off = sub_off[i];
 			t.force = (int16_t)std::min(3, std::max(1, force));
 			const auto &elem = dir == 0 ? elem_by_out : elem_by_in;
+			const bool expd = I.type == 3 && expanded[lane_list(I, i)];
+			if (expd) {
+				t.v_off = expanded_rows(I, i, dir);
+				t.expanded = 1;
+			}
 			if (!swapped[i]) {
 				t.dn = I.type == 3 ? (int64_t)mvals[I.neutron_list].size() : 0;
 				t.nn_in = (int32_t)op->block_nn[bin];
@@ -1044,6 +1096,8 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				if (I.type == 3) {
 					t.n_base = (uint32_t)elem[I.neutron_list];
 					t.n_len = (uint32_t)ln;
+					if (expd)
+						t.dn = (int64_t)ln;
 				} else if (I.type == 1)
 					t.n_base = (uint32_t)rows[I.neutron_list];
 				if (I.type != 1)
@@ -1055,7 +1109,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				t.nn_in = (int32_t)op->block_np[bin];
 				if (I.type == 3) {
 					t.type = 3;
-					t.dn = (int64_t)mvals[I.proton_list].size();
+					t.dn = expd ? (int64_t)lp : (int64_t)mvals[I.proton_list].size();
 					t.n_base = (uint32_t)elem[I.proton_list];
 					t.n_len = (uint32_t)lp;
 					t.p_rows = (uint32_t)rows[I.neutron_list];
@@ -1067,6 +1121,8 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			}
 		}
 	}
+
+	op->compact_elements = (int64_t)W.size();
 
 	// --- host copies for the planner; virtual instructions of a block
 	// sorted by element sub-block so neighbouring tasks share elements ---
