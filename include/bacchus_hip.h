@@ -173,7 +173,7 @@ typedef struct bacchus_hip_batch_tuning {
 	int32_t tiles_per_cu;     /* tile size: total work / (CUs * this); 0 = 16 */
 	int32_t max_tile_vectors; /* vectors per tile, power of two <= 64; 0 = 64 */
 	int32_t workgroups_per_cu; /* persistent grid; 0 = 8                    */
-	int32_t tile_words;       /* FP32 LDS words per tile, <= 16384; 0 = 8192 */
+	int32_t tile_words;       /* FP64 LDS sums per tile, <= 8192; 0 = 4096   */
 } bacchus_hip_batch_tuning_t;
 
 int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops,

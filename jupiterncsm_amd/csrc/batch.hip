@@ -23,8 +23,9 @@ namespace bacchus {
 namespace {
 
 constexpr uint32_t kMask = 0x7FFFFFFFu;
-constexpr int64_t kBatchTileWords = 16384;  // FP32 words of LDS per tile (64 KiB)
+constexpr int64_t kBatchTileWords = 8192;  // FP64 sums of LDS per tile (64 KiB)
 constexpr int kMaxVectors = 256;
+constexpr int64_t kMinTileRows = 8;
 
 struct BatchOp {
 	const TileWork *tiles;
@@ -68,12 +69,13 @@ __device__ __forceinline__ void fma4(float4 &a, float v, const float4 &x)
 	a.w += v * x.w;
 }
 
-__device__ __forceinline__ void add4_lds(float *t, const float4 &s, const float4 &v)
+// the LDS tile holds FP64 sums (ds_add_f64)
+__device__ __forceinline__ void add4_lds(double *t, const float4 &s, const float4 &v)
 {
-	atomicAdd(t + 0, s.x * v.x);
-	atomicAdd(t + 1, s.y * v.y);
-	atomicAdd(t + 2, s.z * v.z);
-	atomicAdd(t + 3, s.w * v.w);
+	atomicAdd(t + 0, (double)(s.x * v.x));
+	atomicAdd(t + 1, (double)(s.y * v.y));
+	atomicAdd(t + 2, (double)(s.z * v.z));
+	atomicAdd(t + 3, (double)(s.w * v.w));
 }
 
 template <typename VT> struct Load;
@@ -102,7 +104,7 @@ template <> struct Load<uint16_t> {
 // N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
 __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
-			   float *tile)
+			   double *tile)
 {
 	const int bt = 1 << lbt;
 	const int lq = lbt - 2;       // log2(lanes per slot)
@@ -112,7 +114,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int twords = direct ? 0 : (nrow * nn) << lbt;
 	float *ys = w.space ? o.yT : o.y;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x)
-		tile[t] = 0.f;
+		tile[t] = 0.0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -134,94 +136,80 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 		const float *xb = xs + (size_t)I.x_off * B + (bvalid ? b : 0);
 		const uint32_t nn_in = (uint32_t)I.nn_in;
 		if (I.type == 3) {
+			// every lane holds kLaneEntries lane-side entries (slot
+			// chunks), so a broadcast row-side entry puts 2 kLaneEntries
+			// independent loads in flight
 			const VT *vblk = V + I.v_off;
 			const uint32_t dn = (uint32_t)I.dn;
-			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 2 * G) {
-				const uint32_t eA = e0 + slot, eB = e0 + G + slot;
-				const bool vA = eA < T.e1 && bvalid, vB = eB < T.e1 && bvalid;
-				const uint32_t kA = I.n_base + (eA < T.e1 ? eA : T.e0);
-				const uint32_t kB = I.n_base + (eB < T.e1 ? eB : T.e0);
-				const uint2 nA = o.ent[kA], nB = o.ent[kB];
-				// element column: compact index, or lane-list position for
-				// expanded element rows
-				const uint32_t cA = I.expanded ? kA - I.n_base : nA.y & kMask;
-				const uint32_t cB = I.expanded ? kB - I.n_base : nB.y & kMask;
-				const uint32_t oA = o.keys[kA], oB = o.keys[kB];
-				const float sgA = sign_of(1.f, nA.y), sgB = sign_of(1.f, nB.y);
-				const float4 sA = make_float4(sgA * sc.x, sgA * sc.y, sgA * sc.z, sgA * sc.w);
-				const float4 sB = make_float4(sgB * sc.x, sgB * sc.y, sgB * sc.z, sgB * sc.w);
-				float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, c0 = a0, c1 = a0;
+			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += kLaneEntries * G) {
+				bool vl[kLaneEntries];
+				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
+				float sgn[kLaneEntries];
+				float4 acc[kLaneEntries];
+#pragma unroll
+				for (int c = 0; c < kLaneEntries; c++) {
+					const uint32_t e = e0 + (uint32_t)(c * G + slot);
+					vl[c] = e < T.e1 && bvalid;
+					const uint32_t k = I.n_base + (e < T.e1 ? e : T.e0);
+					const uint2 en = o.ent[k];
+					// element column: compact index, or lane-list
+					// position for expanded element rows
+					col[c] = I.expanded ? k - I.n_base : en.y & kMask;
+					xin[c] = en.x;
+					outc[c] = o.keys[k];
+					sgn[c] = sign_of(1.f, en.y);
+					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+				}
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
 					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
 					const uint2 pl = o.ent[jl];
 					const uint32_t prow = o.keys[jl];
-					uint32_t t = 0;
-					while (t < cnt) {
+					for (uint32_t t = 0; t < cnt; t++) {
 						const int row = (int)__builtin_amdgcn_readlane(prow, t);
 						if (row != cur) {
 							if (cur >= 0) {
 								const int r = cur - w.po_begin;
-								if (vA) {
-									a0.x += a1.x; a0.y += a1.y; a0.z += a1.z; a0.w += a1.w;
-									add4_lds(&tile[((r * nn + (int)oA) << lbt) + q4], sA, a0);
-								}
-								if (vB) {
-									c0.x += c1.x; c0.y += c1.y; c0.z += c1.z; c0.w += c1.w;
-									add4_lds(&tile[((r * nn + (int)oB) << lbt) + q4], sB, c0);
-								}
+#pragma unroll
+								for (int c = 0; c < kLaneEntries; c++)
+									if (vl[c]) {
+										const float4 s4 = make_float4(sgn[c] * sc.x, sgn[c] * sc.y,
+													      sgn[c] * sc.z, sgn[c] * sc.w);
+										add4_lds(&tile[((r * nn + (int)outc[c]) << lbt) + q4], s4,
+											 acc[c]);
+									}
 							}
-							a0 = make_float4(0.f, 0.f, 0.f, 0.f);
-							a1 = a0;
-							c0 = a0;
-							c1 = a0;
+#pragma unroll
+							for (int c = 0; c < kLaneEntries; c++)
+								acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 							cur = row;
 						}
-						const bool pair = t + 1 < cnt &&
-							(int)__builtin_amdgcn_readlane(prow, t + 1) == row;
-						if (pair) {
-							float va[2], vb[2];
-							float4 xa[2], xc[2];
+						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+						const size_t vrow = (size_t)dn * (py & kMask);
+						const size_t xrow = (size_t)nn_in * px;
+						float va[kLaneEntries];
+						float4 xa[kLaneEntries];
 #pragma unroll
-							for (int u = 0; u < 2; u++) {
-								const uint32_t px = __builtin_amdgcn_readlane(pl.x, t + u);
-								const uint32_t py = __builtin_amdgcn_readlane(pl.y, t + u);
-								const size_t vrow = (size_t)dn * (py & kMask);
-								const size_t xrow = (size_t)nn_in * px;
-								va[u] = sign_of(Load<VT>::at(vblk, vrow + cA), py);
-								vb[u] = sign_of(Load<VT>::at(vblk, vrow + cB), py);
-								xa[u] = ld4(xb + (xrow + nA.x) * B);
-								xc[u] = ld4(xb + (xrow + nB.x) * B);
-							}
-							fma4(a0, va[0], xa[0]);
-							fma4(a1, va[1], xa[1]);
-							fma4(c0, vb[0], xc[0]);
-							fma4(c1, vb[1], xc[1]);
-							t += 2;
-						} else {
-							const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
-							const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-							const size_t vrow = (size_t)dn * (py & kMask);
-							const size_t xrow = (size_t)nn_in * px;
-							fma4(a0, sign_of(Load<VT>::at(vblk, vrow + cA), py),
-							     ld4(xb + (xrow + nA.x) * B));
-							fma4(c0, sign_of(Load<VT>::at(vblk, vrow + cB), py),
-							     ld4(xb + (xrow + nB.x) * B));
-							t += 1;
+						for (int c = 0; c < kLaneEntries; c++) {
+							va[c] = Load<VT>::at(vblk, vrow + col[c]);
+							xa[c] = ld4(xb + (xrow + xin[c]) * B);
 						}
+#pragma unroll
+						for (int c = 0; c < kLaneEntries; c++)
+							fma4(acc[c], sign_of(va[c], py), xa[c]);
 					}
 				}
 				if (cur >= 0) {
 					const int r = cur - w.po_begin;
-					if (vA) {
-						a0.x += a1.x; a0.y += a1.y; a0.z += a1.z; a0.w += a1.w;
-						add4_lds(&tile[((r * nn + (int)oA) << lbt) + q4], sA, a0);
-					}
-					if (vB) {
-						c0.x += c1.x; c0.y += c1.y; c0.z += c1.z; c0.w += c1.w;
-						add4_lds(&tile[((r * nn + (int)oB) << lbt) + q4], sB, c0);
-					}
+#pragma unroll
+					for (int c = 0; c < kLaneEntries; c++)
+						if (vl[c]) {
+							const float4 s4 = make_float4(sgn[c] * sc.x, sgn[c] * sc.y,
+										      sgn[c] * sc.z, sgn[c] * sc.w);
+							add4_lds(&tile[((r * nn + (int)outc[c]) << lbt) + q4], s4, acc[c]);
+						}
 				}
 			}
 		} else if (I.type == 2) {
@@ -323,7 +311,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 		const int e = t >> lbt, c = t & (bt - 1);
 		if (b0 + c >= B)
 			continue;
-		const float v = tile[t];
+		const float v = (float)tile[t];
 		float *dst = yblk + (size_t)e * B + c;
 		if (w.shared) {
 			if (v != 0.f)
@@ -340,7 +328,7 @@ __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items, int nitems,
 	int *__restrict__ next)
 {
-	extern __shared__ float tile[];
+	extern __shared__ double tile[];
 	__shared__ int s_item;
 	for (;;) {
 		if (threadIdx.x == 0)
@@ -480,7 +468,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 	const int nops = (int)bt->ops.size();
 	const bacchus_hip_batch_tuning_t &tu = bt->tuning;
 	const int tiles_per_cu = tu.tiles_per_cu > 0 ? tu.tiles_per_cu : 64;
-	const int64_t words = tu.tile_words > 0 ? std::min<int64_t>(kBatchTileWords, tu.tile_words) : 8192;
+	const int64_t words = tu.tile_words > 0 ? std::min<int64_t>(kBatchTileWords, tu.tile_words) : 4096;
 	const int max_vec = tu.max_tile_vectors > 0 ? pow2_floor(std::min(64, tu.max_tile_vectors)) : 64;
 	const int num_cu = bt->ops[0]->num_cu;
 	double work = 0;
@@ -508,9 +496,14 @@ int build_plan(bacchus_hip_batch_t *bt)
 		pp.tile_elems = std::max<int64_t>(words, 4 * widest);
 		pp.bt.resize(2 * nb);
 		for (int64_t slot = 0; slot < 2 * nb; slot++) {
+			// widest vector chunk that still leaves kMinTileRows rows in
+			// the LDS tile: row-side NP entries are few per row, and a pass
+			// over the lane-side entries should serve several of them
 			const int64_t width = slot < nb ? op->block_nn[slot] : op->block_np[slot - nb];
+			const int64_t rows = slot < nb ? op->block_np[slot] : op->block_nn[slot - nb];
+			const int64_t want = std::min<int64_t>(kMinTileRows, rows);
 			int v = vec;
-			while (v > 4 && width * v > pp.tile_elems)
+			while (v > 4 && width * v * want > pp.tile_elems)
 				v >>= 1;
 			pp.bt[slot] = v;
 		}
@@ -553,7 +546,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 	if (!items.empty())
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(BatchItem),
 					    hipMemcpyHostToDevice));
-	bt->lds_bytes = (size_t)max_words * sizeof(float);
+	bt->lds_bytes = (size_t)max_words * sizeof(double);
 	const int wg_per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : 8;
 	bt->grid = (int)std::max<int64_t>(1, std::min<int64_t>(bt->num_items, (int64_t)num_cu * wg_per_cu));
 	return upload_ops(bt);
@@ -804,9 +797,9 @@ int bacchus_hip_batch_autotune(bacchus_hip_batch_t *bt, int32_t reps,
 	BACCHUS_HIP_CHECK(hipEventCreate(&e0));
 	BACCHUS_HIP_CHECK(hipEventCreate(&e1));
 	const int tpc[] = {16, 64};
-	const int vec[] = {16, 64};
+	const int vec[] = {4, 16, 64};
 	const int wpc[] = {4, 8};
-	const int wds[] = {4096, 8192, 16384};
+	const int wds[] = {2048, 4096, 8192};
 	bacchus_hip_batch_tuning_t best_t = bt->tuning;
 	double best_v = 1e300;
 	int rc = 0;
