@@ -170,7 +170,7 @@ def main():
         pts = L.family_points(args.train_points, 1)
         sol = L.solve_family(op32, pts, iterations=args.train_iterations,
                              num_eigenvalues=args.eigenvalues, target="band",
-                             band=args.eigenvalues)
+                             band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues))
         model = L.fit(sol, basis_size=64, steps=3000, target="band")
         log(f"learned start: {args.train_points} batched family solves "
             f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, "
@@ -247,8 +247,9 @@ def main():
                                f"D={D}", "eigenvalues": args.eigenvalues,
                    "tolerance": args.tolerance, "max_iterations": args.iterations,
                    "fast_path": "fp32 elements + fp64 lanczos, full reorth, " +
-                                ("learned start (MLP over 64 stored band vectors, trained "
-                                 f"on {args.train_points} batched family solves)"
+                                ("learned start (MLP over a 64-vector stored basis of "
+                                 "band eigenvectors, trained on "
+                                 f"{args.train_points} batched family solves)"
                                  if use_learned else "random start")},
         "fp64_gpu_seconds": round(t64, 4),
         "fp64_gpu_iterations": r64.iterations,

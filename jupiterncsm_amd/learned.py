@@ -50,10 +50,12 @@ class FamilySolves:
     targets: object             # torch float32 (P, D), unit norm
     iterations: int
     seconds: float
+    ritz: object = None         # torch float32 (keep_ritz * band, D): the
+                                # band's Ritz vectors of the first points
 
 
 def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64, target="ground",
-                 band=5, reference=None, seed=1, keep_targets=True, tolerance=0.0):
+                 band=5, reference=None, seed=1, keep_targets=True, tolerance=0.0, keep_ritz=0):
     """Batched Lanczos over the points with an FP32/BF16 operator.
 
     reference: float64 device vector (D,) that fixes the ground state's
@@ -69,7 +71,7 @@ def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64
         reference = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
         reference /= reference.norm()
     nritz = 1 if target == "ground" else int(band)
-    evs, tg = [], []
+    evs, tg, kept = [], [], []
     t0 = time.time()
     total_it = 0
     for start in range(0, len(points), batch):
@@ -83,6 +85,11 @@ def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64
                                    num_ritz=nritz)
         total_it += it
         evs.append(ev[0])
+        take = min(B, keep_ritz - start) if keep_ritz > start else 0
+        if take > 0:
+            # (r, D, b) -> (b, r, D): the Ritz vectors of each kept point
+            kept.append(ritz[:, :, :take].permute(2, 0, 1).reshape(-1, D)
+                        .to(torch.float32).contiguous())
         if keep_targets:
             if reference is None:
                 reference = ritz[0][:, 0].clone()
@@ -96,9 +103,10 @@ def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64
             tg.append(t.T.to(torch.float32).contiguous())
         del ritz
     targets = torch.cat(tg, 0) if keep_targets else None
+    ritz_kept = torch.cat(kept, 0) if kept else None
     torch.cuda.synchronize()
     return FamilySolves(np.asarray(points), np.concatenate(evs, 0), targets, total_it,
-                        time.time() - t0)
+                        time.time() - t0, ritz_kept)
 
 
 @dataclass
@@ -161,7 +169,11 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
     error in them is the same error in the start vector."""
     import torch
     T = solves.targets
-    U, rank = orthonormal_basis(T[:basis_size].contiguous())
+    # eigenvector continuation: the individual Ritz vectors of the first
+    # points span a band's eigenvectors at nearby (c1, c2) far better than
+    # the band targets themselves do
+    src = solves.ritz if solves.ritz is not None else T
+    U, rank = orthonormal_basis(src[:basis_size].contiguous())
     n = U.shape[0]
     if log:
         log(f"basis rank {rank} of {n}")
