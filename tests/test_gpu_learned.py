@@ -92,3 +92,33 @@ def test_band_target_projects_onto_lowest_states():
             assert ev[0] - 1e-6 * abs(ev[0]) <= rq <= ev[2] + 1e-6 * abs(ev[2])
             # residual lies in the band: |H t|^2 - rq^2 small against the spread
             assert float(ht @ ht) - rq * rq <= (ev[2] - ev[0]) ** 2 + 1e-6 * rq * rq
+
+
+def test_subspace_matrices_match_dense_projection(tmp_path):
+    """Operator and norm matrices over stored vectors (the Ceres step) against
+    the oracle's dense H (2+2 Nmax 4), and the .npy file they are saved in."""
+    print("case subspace matrices", flush=True)
+    torch = torch_dev()
+    from jupiterncsm_amd import subspace
+    from oracle import oracle as O
+    inp = build(SYSTEMS["he4_nmax4"])
+    rng = np.random.default_rng(3)
+    n = 6
+    V = rng.standard_normal((n, inp.dimension)).astype(np.float32)
+    Vd = torch.as_tensor(V, device="cuda")
+    scales = (1.0, 0.9, 1.1)
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.FP32) as op:
+        M = subspace.subspace_operator(op, Vd, scales)
+    N = subspace.norm_matrix(Vd)
+    p = O.Problem.from_inputs(inp)
+    HV = np.stack([O.matvec(p, v.astype(np.float64), scales=scales) for v in V], 0)
+    M_ref = V.astype(np.float64) @ HV.T
+    M_ref = 0.5 * (M_ref + M_ref.T)
+    assert np.abs(M - M_ref).max() <= 2e-6 * np.abs(M_ref).max()
+    assert np.allclose(M, M.T)
+    assert np.allclose(N, V.astype(np.float64) @ V.astype(np.float64).T, rtol=1e-12, atol=1e-9)
+    path = tmp_path / "subspace_operator.npy"
+    subspace.save_numpy_matrix(str(path), M)
+    raw = open(path, "rb").read()
+    assert raw[:6] == b"\x93NUMPY" and raw[6] == 1 and b"'<f8'" in raw[:128]
+    assert np.array_equal(np.load(path, allow_pickle=False), M)
