@@ -84,6 +84,15 @@ void bacchus_hip_operator_free(bacchus_hip_operator_t *op);
 
 int64_t bacchus_hip_operator_dimension(const bacchus_hip_operator_t *op);
 
+/* Evaluation-order generator: replaces the Mars program
+ * (src/Mars/programs/mars.c:64-199).  Reads the CALCBLOCK rows of comb.txt
+ * and writes the BLOCK: rows in Mars's order (1-body blocks dropped, and
+ * the 3-body ones when two_nucleon_forces_only != 0, as its
+ * --only-two-nucleon-forces option). */
+int bacchus_hip_write_evaluation_order(const char *combination_table_file,
+				       int two_nucleon_forces_only,
+				       const char *output_path);
+
 /* Scale factors per force class (1-, 2-, 3-body); default 1, 1, 1.
  * H(c1, c2) = H1 + c1*V2 + c2*V3 without rewriting element blocks
  * (the linear combination src/Vulcan/programs/vulcan.c:114-207 writes). */

@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_batch_matvec", "bacchus_hip_batch_autotune", "bacchus_hip_batch_stats",
     "bacchus_hip_batch_lanczos", "bacchus_hip_basis_gram", "bacchus_hip_basis_project",
     "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
+    "bacchus_hip_write_evaluation_order",
 )
 
 
@@ -166,6 +167,8 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_basis_project.argtypes = [vp, i64, i64, vp, i64, vp]
     lib.bacchus_hip_basis_combine.restype = c_int
     lib.bacchus_hip_basis_combine.argtypes = [vp, i64, i64, vp, vp, vp]
+    lib.bacchus_hip_write_evaluation_order.restype = c_int
+    lib.bacchus_hip_write_evaluation_order.argtypes = [ctypes.c_char_p, c_int, ctypes.c_char_p]
     lib.bacchus_hip_basis_transform.restype = c_int
     lib.bacchus_hip_basis_transform.argtypes = [vp, i64, i64, vp, i64, vp]
     _LIB = lib
@@ -302,6 +305,14 @@ def run_matrix_vector_multiplication(output_vector_base_directory,
     _check(library().bacchus_hip_run_matrix_vector_multiplication(
         str(output_vector_base_directory).encode(), str(input_vector_base_directory).encode(),
         scheduler.handle))
+
+
+def write_evaluation_order(combination_table_file, output_path, two_nucleon_forces_only=False):
+    """Write the BLOCK: evaluation order Mars would write for comb.txt
+    (src/Mars/programs/mars.c:64-199); host only, no GPU needed."""
+    _check(library().bacchus_hip_write_evaluation_order(
+        str(combination_table_file).encode(), int(bool(two_nucleon_forces_only)),
+        str(output_path).encode()))
 
 
 @dataclass

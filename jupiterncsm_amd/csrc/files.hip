@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -61,11 +62,21 @@ bool read_lines(const char *path, std::vector<std::string> &lines)
 	return true;
 }
 
+// One CALCBLOCK row (calculation_blocks.c:31-84); l2 = -1 when absent.
+struct CalcBlock {
+	int64_t num_particles, in, out, l1, l2, v, mults;
+};
+
 struct CombTable {
 	std::vector<int64_t> block_ids, block_np, block_nn;
 	std::map<int64_t, int64_t> block_index;  // id -> dense
 	std::map<int64_t, char> list_type;       // id -> 'n' / 'p'
 	std::map<int64_t, int> matrix_rank;      // id -> 1, 2, 3 body
+	// bytes of every array by id, as get_array_sizes computes them
+	// (combination_table.c:384-401): blocks 8 Np Nn, lists 12 length,
+	// element blocks 8 x their length (matrix_block_setting.c:3-25)
+	std::map<int64_t, int64_t> array_size;
+	std::vector<CalcBlock> calc;
 };
 
 // Sections by title (combination_table.c:419-640): rows of a section run
@@ -99,6 +110,7 @@ int parse_comb(const char *path, CombTable &t)
 		t.block_ids.push_back(id);
 		t.block_np.push_back(atoll(w[4].c_str()));
 		t.block_nn.push_back(atoll(w[5].c_str()));
+		t.array_size[id] = 8 * t.block_np.back() * t.block_nn.back();
 	});
 	section("Conn lists", [&](const std::string &row) {
 		if (row.find("ARRAY:") == std::string::npos)
@@ -109,6 +121,7 @@ int parse_comb(const char *path, CombTable &t)
 			return;
 		}
 		t.list_type[id_of(w[9])] = w[0][0];
+		t.array_size[id_of(w[9])] = 12 * atoll(w[6].c_str());
 	});
 	// cross p-n rows: type = word 4 + word 0, id = word 13; same p/n rows:
 	// type = word 0, id = word 8 (combination_table.c:535-637)
@@ -121,6 +134,7 @@ int parse_comb(const char *path, CombTable &t)
 			return;
 		}
 		t.matrix_rank[id_of(w[13])] = (int)(w[0].size() + w[4].size());
+		t.array_size[id_of(w[13])] = 8 * atoll(w[8].c_str()) * atoll(w[9].c_str());
 	});
 	section("Matrix-elements V (same p/n)", [&](const std::string &row) {
 		if (row.find("ARRAY:") == std::string::npos)
@@ -131,7 +145,38 @@ int parse_comb(const char *path, CombTable &t)
 			return;
 		}
 		t.matrix_rank[id_of(w[8])] = (int)w[0].size();
+		t.array_size[id_of(w[8])] = 8 * atoll(w[4].c_str());
 	});
+	// CALCBLOCK rows anywhere in the file; the multiplication count is the
+	// word two words before "CALCBLOCK:" (calculation_blocks.c:47-76)
+	for (const std::string &row : lines) {
+		const size_t at = row.find("CALCBLOCK:");
+		if (at == std::string::npos)
+			continue;
+		CalcBlock c{};
+		long long a[6];
+		const char *cb = row.c_str() + at;
+		const int n6 = sscanf(cb, "CALCBLOCK:%lld: %lld, %lld, %lld, %lld, %lld", &a[0], &a[1],
+				      &a[2], &a[3], &a[4], &a[5]);
+		if (n6 == 6) {
+			c = {a[0], a[1], a[2], a[3], a[4], a[5], 0};
+		} else if (sscanf(cb, "CALCBLOCK:%lld: %lld, %lld, %lld, %lld", &a[0], &a[1], &a[2],
+				  &a[3], &a[4]) == 5) {
+			c = {a[0], a[1], a[2], a[3], -1, a[4], 0};
+		} else {
+			bad = 2;
+			continue;
+		}
+		const auto before = split_words(row.substr(0, at));
+		if (before.size() < 2) {
+			bad = 2;
+			continue;
+		}
+		c.mults = atoll(before[before.size() - 2].c_str());
+		t.calc.push_back(c);
+	}
+	if (bad == 2)
+		return BACCHUS_FAIL("CALCBLOCK row in %s is not formatted correctly", path);
 	if (bad)
 		return BACCHUS_FAIL("malformed ARRAYMP/ARRAY row in %s", path);
 	if (t.block_ids.empty())
@@ -154,6 +199,55 @@ int read_file(const std::string &path, std::vector<char> &buf)
 	}
 	fclose(f);
 	return 0;
+}
+
+// The calculation-block order of Mars (src/Mars/programs/mars.c:92-170):
+// blocks grouped by particle number, the 1-body ones dropped (and the
+// 3-body ones in two-nucleon mode), then seven stable sorts whose last key
+// is the primary one: heaviest first, then the size of the largest array,
+// the ids of the five arrays ordered by size.
+std::vector<CalcBlock> mars_order(const CombTable &t, bool two_nucleon_only)
+{
+	constexpr uint64_t kNoIndex = ~0ull;  // no_index, global_constants.h:8
+	std::vector<CalcBlock> v;
+	for (int np = 1; np <= 3; np++)
+		for (const CalcBlock &c : t.calc)
+			if (c.num_particles == np)
+				v.push_back(c);
+	std::vector<CalcBlock> need;
+	for (const CalcBlock &c : v)
+		if (c.num_particles == 2 || (!two_nucleon_only && c.num_particles == 3))
+			need.push_back(c);
+	auto size_of = [&](uint64_t id) -> uint64_t {
+		if (id == kNoIndex)
+			return 0;
+		auto it = t.array_size.find((int64_t)id);
+		return it == t.array_size.end() ? 0 : (uint64_t)it->second;
+	};
+	// the five arrays ordered by size with mars.c's own compare-and-swap
+	// sequence (quint_sort, :322-340)
+	auto arrays = [&](const CalcBlock &c) {
+		std::array<uint64_t, 5> a = {(uint64_t)c.in, (uint64_t)c.out, (uint64_t)c.l1,
+					     c.l2 < 0 ? kNoIndex : (uint64_t)c.l2, (uint64_t)c.v};
+		for (int i = 1; i < 5; i++)
+			for (int j = 0; j < i; j++)
+				if (size_of(a[j]) > size_of(a[j + 1]))
+					std::swap(a[j], a[j + 1]);
+		return a;
+	};
+	auto by = [&](auto key) {
+		std::stable_sort(need.begin(), need.end(), [&](const CalcBlock &x, const CalcBlock &y) {
+			return key(x) < key(y);
+		});
+	};
+	by([&](const CalcBlock &c) { return arrays(c)[0]; });
+	by([&](const CalcBlock &c) { return arrays(c)[1]; });
+	by([&](const CalcBlock &c) { return arrays(c)[2]; });
+	by([&](const CalcBlock &c) { return arrays(c)[3]; });
+	by([&](const CalcBlock &c) { return arrays(c)[4]; });
+	by([&](const CalcBlock &c) { return size_of(arrays(c)[4]); });
+	by([&](const CalcBlock &c) { return (uint64_t)(-c.mults); });
+	return need;
 }
 
 }  // namespace
@@ -359,6 +453,32 @@ int bacchus_hip_run_matrix_vector_multiplication(
 	if (rc)
 		return rc;
 	return io(output_vector_base_directory, y, true);
+}
+
+int bacchus_hip_write_evaluation_order(const char *combination_table_file,
+				       int two_nucleon_forces_only, const char *output_path)
+{
+	if (!combination_table_file || !output_path)
+		return BACCHUS_FAIL("NULL argument");
+	CombTable t;
+	if (int rc = parse_comb(combination_table_file, t))
+		return rc;
+	const auto order = mars_order(t, two_nucleon_forces_only != 0);
+	FILE *f = fopen(output_path, "w");
+	if (!f)
+		return BACCHUS_FAIL("Could not open \"%s\" for writing. %s", output_path,
+				    strerror(errno));
+	for (const CalcBlock &c : order) {
+		if (c.l2 < 0)
+			fprintf(f, "BLOCK: %lld %lld %lld %lld\n", (long long)c.in, (long long)c.out,
+				(long long)c.l1, (long long)c.v);
+		else
+			fprintf(f, "BLOCK: %lld %lld %lld %lld %lld\n", (long long)c.in,
+				(long long)c.out, (long long)c.l1, (long long)c.l2, (long long)c.v);
+	}
+	if (fclose(f) != 0)
+		return BACCHUS_FAIL("Could not write \"%s\"", output_path);
+	return 0;
 }
 
 }  // extern "C"

@@ -12,7 +12,13 @@ in the reference's file formats, and stores:
   krylov_conv       Krylov dimension reached by that run
   digest            checksums of the generated arrays (pins the generator)
 
+and, from the compiled Mars program (oracle/_ref/mars_ref,
+src/Mars/programs/mars.c), the evaluation order it writes for each case's
+comb.txt: <name>.mars_order (default mode) and, for the 3-body cases,
+<name>.mars_order_2nf (--only-two-nucleon-forces).
+
 Only runs where the reference tree is present; the fixtures are data.
+`--mars-only` rewrites just the evaluation-order fixtures.
 """
 import os
 import shutil
@@ -28,6 +34,7 @@ sys.path.insert(0, ROOT)
 from jupiterncsm_amd.inputs import SystemConfig, build  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "bacchus_ref")
+MARS = os.path.join(ROOT, "oracle", "_ref", "mars_ref")
 
 CASES = {
     "he4_nmax4": (SystemConfig(2, 2, 4, max_body=2, seed=1), 60),
@@ -98,5 +105,29 @@ def main():
             shutil.rmtree(work, ignore_errors=True)
 
 
+def mars_orders():
+    out_dir = os.path.dirname(os.path.abspath(__file__))
+    for name, (cfg, _) in CASES.items():
+        inp = build(cfg)
+        work = tempfile.mkdtemp(prefix=f"mars_{name}_")
+        try:
+            inp.write(work)
+            comb = os.path.join(work, "comb.txt")
+            modes = [("mars_order", [])]
+            if cfg.max_body >= 3:
+                modes.append(("mars_order_2nf", ["--only-two-nucleon-forces"]))
+            for suffix, extra in modes:
+                target = os.path.join(out_dir, f"{name}.{suffix}")
+                r = subprocess.run([MARS, comb, target, *extra], capture_output=True, text=True,
+                                   cwd=work)
+                if r.returncode != 0:
+                    raise RuntimeError(r.stderr[-2000:])
+                print(name, suffix, sum(1 for _ in open(target)), "blocks")
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+
+
 if __name__ == "__main__":
-    main()
+    if "--mars-only" not in sys.argv:
+        main()
+    mars_orders()
