@@ -6,13 +6,21 @@ elements and the one-body term 20(N+1.5) MeV: D = 197,822.
 
 A step is one solve of the lowest 5 eigenvalues to 1e-6 relative accuracy
 against FP64 reference eigenvalues (the FP64 GPU path run to 300 Lanczos
-iterations with full reorthogonalization) with the fast path: FP32-stored
-matrix elements with FP32 products and accumulation, FP64 Lanczos
-recurrence, full FP64 reorthogonalization and the FP64 tridiagonal solve on
-the GPU, started from the learned start vector (an MLP over a stored basis,
-trained beforehand on batched solves of the family H(c1, c2) = H1 + c1 V2 +
-c2 V3, untimed).  The plain FP64 GPU path (random start) is timed to the
-same criterion beside it.
+iterations with full reorthogonalization) with the fast path:
+
+- FP32-stored matrix elements with FP32 products and accumulation;
+- block Lanczos, 8 vectors per block, each block step one batched launch,
+  FP64 inner products and block tridiagonal solve, stopping on its own
+  residual bound |r|^2 / gap <= 1e-6 |theta|;
+- started from the learned stored basis: 64 FP32 vectors spanning band
+  eigenvectors of batched solves of the family H(c1, c2) = H1 + c1 V2 +
+  c2 V3, onto which H1, V2, V3 are projected once; the start block is the
+  Ritz vectors of H(c1, c2) in that span (training and projections
+  untimed).
+
+The plain FP64 GPU path (single-vector Lanczos, random start) is timed to
+the target criterion beside it, and so is the single-vector FP32 Lanczos
+from the MLP-predicted start vector over the same basis.
 
 Other modes (one JSON line each):
   --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
@@ -57,7 +65,9 @@ def parse():
     ap.add_argument("--no-learned", action="store_true",
                     help="fast path from a random start instead of the learned one")
     ap.add_argument("--start-mix", type=float, default=1e-4,
-                    help="relative size of the random component mixed into the learned start")
+                    help="relative size of the random component mixed into the MLP start")
+    ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
+    ap.add_argument("--max-blocks", type=int, default=16)
     ap.add_argument("--batched", action="store_true", help="batched 64-Hamiltonian sweep")
     ap.add_argument("--learned", action="store_true", help="learned-start measurement")
     ap.add_argument("--instances", type=int, default=64)
@@ -163,7 +173,8 @@ def main():
     log(f"reference eigenvalues ({ref.iterations} its, {time.time() - t0:.2f}s): {targets}")
 
     # --- learned start: trained on batched solves of the family (untimed) -
-    model = None
+    model = fam = bt = None
+    train_s = 0.0
     if not args.no_learned:
         from jupiterncsm_amd import learned as L
         t0 = time.time()
@@ -172,43 +183,75 @@ def main():
                              num_eigenvalues=args.eigenvalues, target="band",
                              band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues))
         model = L.fit(sol, basis_size=64, steps=3000, target="band")
+        t1 = time.time()
+        fam = L.ProjectedFamily.build(op64, model.basis, model.rank)
+        train_s = time.time() - t0
         log(f"learned start: {args.train_points} batched family solves "
-            f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, "
-            f"total {time.time() - t0:.1f}s")
+            f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, basis rank {model.rank}, "
+            f"projections {time.time() - t1:.1f}s, total {train_s:.1f}s")
+        bt = bacchus.Batch([op32], [args.block])
+        bt.set_scales(0, np.ones((args.block, 3)))
+        tuned = bt.autotune(3)
+        log(f"block matvec ({args.block} vectors) autotuned: {tuned['ms']:.2f} ms")
 
-    def solve(op, seed, learned=False):
+    def timed(fn):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        kw = dict(num_eigenvalues=args.eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
-                  convergence=bacchus.CONVERGE_TARGETS, tolerance=args.tolerance,
-                  target_eigenvalues=targets, check_every=1)
-        sv = model.start_vector(1.0, 1.0, mix=args.start_mix) if learned else None
-        if sv is not None:
-            r = bacchus.diagonalize(op, args.iterations, start=bacchus.START_GIVEN,
-                                    start_vector=sv, **kw)
-        else:
-            r = bacchus.diagonalize(op, args.iterations, start=bacchus.START_RANDOM,
-                                    seed=seed, **kw)
+        r = fn()
         torch.cuda.synchronize()
         return time.perf_counter() - t, r
 
-    # plain FP64 GPU path to the same criterion
-    t64, r64 = solve(op64, 2)
-    t64, r64 = solve(op64, 2)
-    log(f"fp64 path: {t64:.3f}s, {r64.iterations} its, errors "
-        f"{np.abs(r64.eigenvalues - targets) / np.abs(targets)}")
+    def single(op, seed, start=None):
+        kw = dict(num_eigenvalues=args.eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
+                  convergence=bacchus.CONVERGE_TARGETS, tolerance=args.tolerance,
+                  target_eigenvalues=targets, check_every=1)
+        if start is not None:
+            return bacchus.diagonalize(op, args.iterations, start=bacchus.START_GIVEN,
+                                       start_vector=start, **kw)
+        return bacchus.diagonalize(op, args.iterations, start=bacchus.START_RANDOM, seed=seed,
+                                   **kw)
 
-    use_learned = model is not None
+    def mlp_solve():
+        sv = model.start_vector(1.0, 1.0, mix=args.start_mix)
+        return single(op32, 2, sv)
+
+    def block_solve():
+        X = fam.start_block(1.0, 1.0, args.block)
+        return bt.block_lanczos(args.max_blocks, args.eigenvalues, start_block=X,
+                                convergence=bacchus.CONVERGE_RESIDUALS,
+                                tolerance=args.tolerance)
+
+    def rel(ev):
+        return np.abs(np.asarray(ev) - targets) / np.abs(targets)
+
+    # plain FP64 GPU path to the target criterion
+    t64, r64 = timed(lambda: single(op64, 2))
+    t64, r64 = timed(lambda: single(op64, 2))
+    log(f"fp64 path: {t64:.3f}s, {r64.iterations} its, errors {rel(r64.eigenvalues)}")
+
+    mlp = None
+    if model is not None:
+        timed(mlp_solve)
+        tm, rm = timed(mlp_solve)
+        mlp = {"seconds": round(tm, 4), "iterations": rm.iterations,
+               "speedup_vs_fp64_gpu": round(t64 / tm, 3),
+               "relative_errors": [float(v) for v in rel(rm.eigenvalues)]}
+        log(f"fp32 single-vector Lanczos from the MLP start: {tm:.3f}s, {rm.iterations} its, "
+            f"errors {rel(rm.eigenvalues)}")
+
+    fast = block_solve if fam is not None else (lambda: single(op32, 2))
     for _ in range(args.warmup):
-        solve(op32, 2, use_learned)
+        timed(fast)
     times, last = [], None
     for _ in range(args.steps):
-        dt, last = solve(op32, 2, use_learned)
+        dt, last = timed(fast)
         times.append(dt)
     t_fast = float(np.mean(times))
-    errs = np.abs(last.eigenvalues - targets) / np.abs(targets)
-    converged = last.iterations < args.iterations or bool(np.all(errs <= args.tolerance))
-    log(f"fast path: {t_fast:.3f}s, {last.iterations} its, errors {errs}")
+    errs = rel(last.eigenvalues)
+    converged = bool(np.all(errs <= args.tolerance))
+    steps_taken = last.blocks if fam is not None else last.iterations
+    log(f"fast path: {t_fast:.4f}s, {steps_taken} {'blocks' if fam is not None else 'its'}, "
+        f"errors {errs}")
 
     # --- CPU oracle on the host cores (bounded sample) -------------------
     cpu = None
@@ -230,6 +273,14 @@ def main():
                          "Lanczos vector work not included"}
         log(f"cpu oracle matvec {cpu_mv:.2f}s on {cores} threads")
 
+    if fam is not None:
+        fast_desc = (f"fp32 elements; block Lanczos ({args.block} vectors per batched launch, "
+                     "FP64 inner products and block tridiagonal solve, stops on its residual "
+                     "bound); start block = Ritz vectors of H(c1,c2) in a 64-vector stored "
+                     "basis of band eigenvectors from "
+                     f"{args.train_points} batched family solves")
+    else:
+        fast_desc = "fp32 elements + fp64 single-vector lanczos, full reorth, random start"
     line = {
         "metric": METRIC,
         "value": round(t_fast, 4),
@@ -246,20 +297,18 @@ def main():
         "config": {"workload": f"{args.workload}: 3p+3n Nmax6 M=0 +parity, 2+3-body, "
                                f"D={D}", "eigenvalues": args.eigenvalues,
                    "tolerance": args.tolerance, "max_iterations": args.iterations,
-                   "fast_path": "fp32 elements + fp64 lanczos, full reorth, " +
-                                ("learned start (MLP over a 64-vector stored basis of "
-                                 "band eigenvectors, trained on "
-                                 f"{args.train_points} batched family solves)"
-                                 if use_learned else "random start")},
+                   "fast_path": fast_desc},
         "fp64_gpu_seconds": round(t64, 4),
         "fp64_gpu_iterations": r64.iterations,
-        "fast_iterations": last.iterations,
+        "fast_steps": steps_taken,
+        "fast_matvecs": steps_taken * (args.block if fam is not None else 1),
         "speedup_vs_fp64_gpu": round(t64 / t_fast, 3),
         "converged": converged,
+        "training_seconds": round(train_s, 2),
+        "mlp_start": mlp,
         "eigenvalues_fp64_reference": [float(v) for v in targets],
         "relative_errors_fast": [float(v) for v in errs],
-        "relative_errors_fp64": [float(v) for v in
-                                 np.abs(r64.eigenvalues - targets) / np.abs(targets)],
+        "relative_errors_fp64": [float(v) for v in rel(r64.eigenvalues)],
         "matvec_fp64_ms": round(mv_ms, 4),
         "matvec_fp32_ms": round(mv32_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

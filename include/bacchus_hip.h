@@ -133,7 +133,10 @@ enum bacchus_hip_reorth { BACCHUS_HIP_REORTH_REFERENCE = 0, /* MGS vs
 			  BACCHUS_HIP_REORTH_NONE = 2 };
 enum bacchus_hip_convergence { BACCHUS_HIP_CONVERGE_EIGENVALUES = 0,
 			       BACCHUS_HIP_CONVERGE_NONE = 2,
-			       BACCHUS_HIP_CONVERGE_TARGETS = 3 };
+			       BACCHUS_HIP_CONVERGE_TARGETS = 3,
+			       /* block solver: |r_k|^2 / gap_k <= tol |theta_k|
+				  for the lowest num_eigenvalues Ritz pairs */
+			       BACCHUS_HIP_CONVERGE_RESIDUALS = 4 };
 
 typedef struct bacchus_hip_lanczos_settings {
 	int64_t max_num_iterations;
@@ -236,6 +239,39 @@ int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *batch,
 			      const bacchus_hip_batch_lanczos_settings_t *settings,
 			      double *eigenvalues, double *const *ritz_vectors,
 			      bacchus_hip_lanczos_result_t *result);
+
+/* Block Lanczos over a batch of one operator (no counterpart in the
+ * reference, whose Lanczos is single-vector, src/Bacchus/lanczos/
+ * lanczos.c:74-253).  The batch's b vectors form the block (1 <= b <= 16);
+ * every vector of the batch must carry the same scales.  One batched
+ * launch per block step; Krylov blocks stored FP32 [D][b], inner products
+ * in FP64, classical block Gram-Schmidt against all blocks twice, block
+ * tridiagonal eigensystem in FP64 on the host.  Krylov dimension
+ * b * max_num_blocks <= 256. */
+typedef struct bacchus_hip_block_lanczos_settings {
+	int32_t max_num_blocks;
+	int32_t num_eigenvalues;  /* 1..b                                      */
+	int32_t convergence;      /* TARGETS, RESIDUALS or NONE                */
+	int32_t pad;
+	double tolerance;         /* relative                                  */
+	const double *target_eigenvalues; /* CONVERGE_TARGETS                  */
+	const float *start_block; /* device FP32 [D][b]; NULL = seeded random  */
+	uint64_t seed;
+} bacchus_hip_block_lanczos_settings_t;
+
+/* eigenvalues, residuals: host, num_eigenvalues each (residual = |H y - theta
+ * y| of the Ritz pair, from the recurrence); ritz_vectors: NULL or device
+ * FP64 [num_eigenvalues][D]; result->iterations = block steps taken. */
+int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
+			      const bacchus_hip_block_lanczos_settings_t *settings,
+			      double *eigenvalues, double *residuals,
+			      double *ritz_vectors,
+			      bacchus_hip_lanczos_result_t *result);
+
+/* Dense symmetric eigensystem on the host (cyclic Jacobi, FP64), the
+ * solver of the block tridiagonal matrix: A n x n row-major (symmetrized),
+ * w ascending, V (may be NULL) eigenvector k in column k.  No GPU needed. */
+int bacchus_hip_symmetric_eigen(int64_t n, const double *A, double *w, double *V);
 
 /* Stored-basis helpers of the learned start (no counterpart in the
  * reference).  U holds n FP32 vectors of dimension D ([n][D]).

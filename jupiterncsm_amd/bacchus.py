@@ -35,7 +35,7 @@ FP64, FP32, BF16 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16}
 START_E0, START_RANDOM, START_GIVEN = 0, 1, 2
 REORTH_REFERENCE, REORTH_FULL, REORTH_NONE = 0, 1, 2
-CONVERGE_EIGENVALUES, CONVERGE_NONE, CONVERGE_TARGETS = 0, 2, 3
+CONVERGE_EIGENVALUES, CONVERGE_NONE, CONVERGE_TARGETS, CONVERGE_RESIDUALS = 0, 2, 3, 4
 
 EXPORTED_SYMBOLS = (
     "bacchus_hip_operator_create", "bacchus_hip_operator_load", "bacchus_hip_operator_free",
@@ -49,7 +49,8 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_batch_matvec", "bacchus_hip_batch_autotune", "bacchus_hip_batch_stats",
     "bacchus_hip_batch_lanczos", "bacchus_hip_basis_gram", "bacchus_hip_basis_project",
     "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
-    "bacchus_hip_write_evaluation_order",
+    "bacchus_hip_write_evaluation_order", "bacchus_hip_block_lanczos",
+    "bacchus_hip_symmetric_eigen",
 )
 
 
@@ -90,6 +91,13 @@ class _BatchLanczosSettings(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("eigenvalue_tolerance", ctypes.c_double),
                 ("check_every", ctypes.c_int32), ("num_ritz_vectors", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("start_vectors", ctypes.c_void_p)]
+
+
+class _BlockLanczosSettings(ctypes.Structure):
+    _fields_ = [("max_num_blocks", ctypes.c_int32), ("num_eigenvalues", ctypes.c_int32),
+                ("convergence", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("target_eigenvalues", ctypes.c_void_p),
+                ("start_block", ctypes.c_void_p), ("seed", ctypes.c_uint64)]
 
 
 class _LanczosResult(ctypes.Structure):
@@ -169,6 +177,11 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_basis_combine.argtypes = [vp, i64, i64, vp, vp, vp]
     lib.bacchus_hip_write_evaluation_order.restype = c_int
     lib.bacchus_hip_write_evaluation_order.argtypes = [ctypes.c_char_p, c_int, ctypes.c_char_p]
+    lib.bacchus_hip_block_lanczos.restype = c_int
+    lib.bacchus_hip_block_lanczos.argtypes = [vp, ctypes.POINTER(_BlockLanczosSettings), vp, vp,
+                                              vp, ctypes.POINTER(_LanczosResult)]
+    lib.bacchus_hip_symmetric_eigen.restype = c_int
+    lib.bacchus_hip_symmetric_eigen.argtypes = [i64, vp, vp, vp]
     lib.bacchus_hip_basis_transform.restype = c_int
     lib.bacchus_hip_basis_transform.argtypes = [vp, i64, i64, vp, i64, vp]
     _LIB = lib
@@ -325,6 +338,28 @@ class LanczosResult:
     total_seconds: float
 
 
+@dataclass
+class BlockLanczosResult:
+    eigenvalues: np.ndarray     # lowest Ritz values, ascending
+    residuals: np.ndarray       # |H y - theta y| of those Ritz pairs
+    blocks: int                 # block steps (batched matvecs) taken
+    matvec_seconds: float
+    total_seconds: float
+
+
+def symmetric_eigen(A):
+    """Eigenvalues (ascending) and eigenvectors (columns) of the symmetric
+    matrix A with the library's host Jacobi solver (no GPU needed)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise BacchusError("A must be square")
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    _check(library().bacchus_hip_symmetric_eigen(n, _ptr(A), _ptr(w), _ptr(V)))
+    return w, V
+
+
 def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
                 start=START_E0, reorthogonalization=REORTH_REFERENCE,
                 convergence=CONVERGE_NONE, tolerance=1e-7, seed=1, start_vector=None,
@@ -449,6 +484,43 @@ class Batch:
             out.append(ev[pos:pos + b * num_eigenvalues].reshape(b, num_eigenvalues))
             pos += b * num_eigenvalues
         return out, int(res.iterations), float(res.total_seconds)
+
+    def block_lanczos(self, max_num_blocks, num_eigenvalues, *, start_block=None,
+                      convergence=CONVERGE_RESIDUALS, tolerance=1e-6, targets=None, seed=1,
+                      ritz_vectors=None):
+        """Block Lanczos over this batch's single operator, the block being
+        its b vectors (all with the same scales).  start_block: float32
+        device tensor (D, b) or None (seeded random); ritz_vectors: float64
+        device tensor (num_eigenvalues, D) or None.  Returns
+        BlockLanczosResult."""
+        import torch
+        if len(self.operators) != 1:
+            raise BacchusError("block Lanczos needs a batch of one operator")
+        D, b = self.operators[0].dimension, self.num_vectors[0]
+        sb = rv = None
+        if start_block is not None:
+            sb = _tensor(start_block, torch.float32, "start_block")
+            if tuple(start_block.shape) != (D, b):
+                raise BacchusError(f"start_block must be ({D}, {b})")
+        if ritz_vectors is not None:
+            rv = _tensor(ritz_vectors, torch.float64, "ritz_vectors")
+            if tuple(ritz_vectors.shape) != (num_eigenvalues, D):
+                raise BacchusError(f"ritz_vectors must be ({num_eigenvalues}, {D})")
+        tg = None
+        if targets is not None:
+            tg = np.ascontiguousarray(targets, dtype=np.float64)
+            if tg.size < num_eigenvalues:
+                raise BacchusError(f"need {num_eigenvalues} target eigenvalues")
+        st = _BlockLanczosSettings(int(max_num_blocks), int(num_eigenvalues), int(convergence), 0,
+                                   float(tolerance), _ptr(tg) if tg is not None else None,
+                                   sb, int(seed))
+        ev = np.zeros(num_eigenvalues)
+        rs = np.zeros(num_eigenvalues)
+        res = _LanczosResult()
+        _check(library().bacchus_hip_block_lanczos(
+            self.handle, ctypes.byref(st), _ptr(ev), _ptr(rs), rv, ctypes.byref(res)))
+        return BlockLanczosResult(ev, rs, int(res.iterations), float(res.matvec_seconds),
+                                  float(res.total_seconds))
 
     def close(self):
         if self._h is not None and self._h.value:

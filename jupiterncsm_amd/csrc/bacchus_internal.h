@@ -113,6 +113,10 @@ struct Plan {
 	int64_t max_tile_elems = 1;
 };
 
+// Shape of a batch: operators, vectors of operator 0, its dimension.
+int batch_shape(const bacchus_hip_batch_t *batch, int32_t *num_ops, int32_t *num_vectors,
+		int64_t *dimension);
+
 // Host tridiagonal helpers (diag d, off-diagonal e, order k): lowest
 // `count` eigenvalues by Sturm bisection, and the eigenvector of one
 // eigenvalue by inverse iteration (component 0 >= 0).

@@ -610,6 +610,18 @@ int use_internal(bacchus_hip_batch_t *bt)
 }
 
 }  // namespace
+
+int batch_shape(const bacchus_hip_batch_t *bt, int32_t *num_ops, int32_t *num_vectors,
+		int64_t *dimension)
+{
+	if (!bt || bt->ops.empty())
+		return BACCHUS_FAIL("empty batch");
+	*num_ops = (int32_t)bt->ops.size();
+	*num_vectors = bt->B[0];
+	*dimension = bt->ops[0]->dimension;
+	return 0;
+}
+
 }  // namespace bacchus
 
 extern "C" {

@@ -1,0 +1,572 @@
+// block.hip - block Lanczos over the batched matvec, started from a given
+// block of vectors (no counterpart in the reference, whose Lanczos is the
+// single-vector recurrence of src/Bacchus/lanczos/lanczos.c:74-253).
+//
+// The b vectors of a block go through one batched launch per step (FP32 or
+// BF16 elements, FP32 vectors).  Krylov blocks are stored in FP32 as
+// [D][b] (vector index fastest, the batched layout); every inner product
+// accumulates in FP64 and the small dense algebra (block tridiagonal
+// matrix, its eigensystem, Gram matrices) runs in FP64 on the host.
+// Orthogonalization is classical block Gram-Schmidt against every stored
+// block, twice, followed by an orthonormalization of the block through the
+// eigensystem of its Gram matrix (SVQB); columns without a direction of
+// their own are refilled with seeded random vectors.
+#include "bacchus_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace bacchus {
+namespace {
+
+constexpr int kInnerChunks = 128;
+constexpr int kMaxBlock = 16;
+constexpr int kMaxKrylov = 256;
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t z)
+{
+	z += 0x9e3779b97f4a7c15ULL;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+	return z ^ (z >> 31);
+}
+
+// partial[(j * chunks + c) * b * b + p * b + q] = sum over the rows r of
+// chunk c of Q_j[r][p] * W[r][q]; Q holds nblk blocks [D][b]
+__global__ __launch_bounds__(256) void block_inner(const float *__restrict__ Q,
+						   const float *__restrict__ W, int64_t D, int b,
+						   double *__restrict__ partial)
+{
+	__shared__ double sh[256];
+	const int j = blockIdx.y;
+	const int bb = b * b;
+	const int groups = 256 / bb;
+	const int t = threadIdx.x;
+	const int g = t / bb, pq = t % bb, p = pq / b, q = pq % b;
+	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
+	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
+	const float *Qj = Q + (int64_t)j * D * b;
+	double s = 0;
+	if (g < groups)
+		for (int64_t r = r0 + g; r < r1; r += groups)
+			s += (double)Qj[r * b + p] * (double)W[r * b + q];
+	sh[t] = s;
+	__syncthreads();
+	if (t < bb) {
+		double v = 0;
+		for (int k = 0; k < groups; k++)
+			v += sh[t + k * bb];
+		partial[((int64_t)j * gridDim.x + blockIdx.x) * bb + t] = v;
+	}
+}
+
+// out[r][q] = in[r][q] (or 0) + sum_{j, p} Q_j[r][p] C[(j b + p) m + q] for
+// q < m; in may alias out (each thread reads its row before writing it).
+// row_major: out is [m][D] instead of [D][m].
+template <typename OT>
+__global__ __launch_bounds__(256) void block_combine(const float *__restrict__ Q, int nblk, int b,
+						     int64_t D, const double *__restrict__ C, int m,
+						     const float *in, OT *out, int row_major)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= D)
+		return;
+	double acc[kMaxBlock];
+#pragma unroll
+	for (int q = 0; q < kMaxBlock; q++)
+		acc[q] = (in && q < m) ? (double)in[r * m + q] : 0.0;
+	for (int j = 0; j < nblk; j++) {
+		const float *row = Q + ((int64_t)j * D + r) * b;
+		for (int p = 0; p < b; p++) {
+			const double v = (double)row[p];
+			const double *c = C + ((int64_t)j * b + p) * m;
+#pragma unroll
+			for (int q = 0; q < kMaxBlock; q++)
+				if (q < m)
+					acc[q] += v * c[q];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < kMaxBlock; q++)
+		if (q < m) {
+			if (row_major)
+				out[(int64_t)q * D + r] = (OT)acc[q];
+			else
+				out[r * m + q] = (OT)acc[q];
+		}
+}
+
+// column col of W [D][b] <- uniform [-1, 1) keyed by (seed, row)
+__global__ void block_random_column(float *W, int64_t D, int b, int col, uint64_t seed)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < D)
+		W[r * b + col] =
+			(float)(2.0 * double(splitmix(seed ^ splitmix((uint64_t)r)) >> 11) * 0x1.0p-53 - 1.0);
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// Cyclic Jacobi (two-sided rotations chosen as in Golub & Van Loan,
+// Matrix Computations, Alg. 8.5.2): A = V diag(w) V^T, w ascending,
+// eigenvector k in column k of V (row-major n x n).
+void jacobi_eigen(int n, std::vector<double> A, std::vector<double> &w, std::vector<double> &V)
+{
+	std::vector<double> Z((size_t)n * n, 0.0);
+	for (int i = 0; i < n; i++)
+		Z[(size_t)i * n + i] = 1.0;
+	auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+	double norm2 = 0;
+	for (double v : A)
+		norm2 += v * v;
+	for (int sweep = 0; sweep < 64; sweep++) {
+		double off = 0;
+		for (int i = 0; i < n; i++)
+			for (int j = i + 1; j < n; j++)
+				off += a(i, j) * a(i, j);
+		if (off <= 1e-32 * norm2 || off == 0.0)
+			break;
+		for (int p = 0; p < n; p++)
+			for (int q = p + 1; q < n; q++) {
+				const double apq = a(p, q);
+				if (apq == 0.0)
+					continue;
+				const double tau = (a(q, q) - a(p, p)) / (2.0 * apq);
+				const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+				const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+				// A <- J^T A J with J the rotation in the (p, q) plane
+				for (int k = 0; k < n; k++) {
+					const double akp = a(k, p), akq = a(k, q);
+					a(k, p) = c * akp - s * akq;
+					a(k, q) = s * akp + c * akq;
+				}
+				for (int k = 0; k < n; k++) {
+					const double apk = a(p, k), aqk = a(q, k);
+					a(p, k) = c * apk - s * aqk;
+					a(q, k) = s * apk + c * aqk;
+				}
+				for (int k = 0; k < n; k++) {
+					double *z = &Z[(size_t)k * n];
+					const double zp = z[p], zq = z[q];
+					z[p] = c * zp - s * zq;
+					z[q] = s * zp + c * zq;
+				}
+			}
+	}
+	std::vector<int> order(n);
+	std::iota(order.begin(), order.end(), 0);
+	std::stable_sort(order.begin(), order.end(), [&](int i, int j) { return a(i, i) < a(j, j); });
+	w.assign(n, 0.0);
+	V.assign((size_t)n * n, 0.0);
+	for (int k = 0; k < n; k++) {
+		w[k] = a(order[k], order[k]);
+		for (int i = 0; i < n; i++)
+			V[(size_t)i * n + k] = Z[(size_t)i * n + order[k]];
+	}
+}
+
+struct Block {
+	int64_t D = 0;
+	int b = 0;
+	int max_blocks = 0;
+	float *Q = nullptr;      // max_blocks + 1 blocks [D][b]
+	float *X = nullptr;      // matvec input  [D][b]
+	float *W = nullptr;      // matvec output [D][b]
+	float *Y = nullptr;      // scratch       [D][b]
+	double *dC = nullptr;    // combination coefficients
+	double *dpart = nullptr; // inner-product partial sums
+	std::vector<double> hpart;
+	double matvec_ms = 0;
+
+	~Block()
+	{
+		(void)hipFree(Q);
+		(void)hipFree(X);
+		(void)hipFree(W);
+		(void)hipFree(Y);
+		(void)hipFree(dC);
+		(void)hipFree(dpart);
+	}
+	float *blk(int j) const { return Q + (int64_t)j * D * b; }
+
+	int alloc()
+	{
+		const size_t vec = (size_t)D * b * sizeof(float);
+		BACCHUS_HIP_CHECK(hipMalloc(&Q, vec * (max_blocks + 1)));
+		BACCHUS_HIP_CHECK(hipMalloc(&X, vec));
+		BACCHUS_HIP_CHECK(hipMalloc(&W, vec));
+		BACCHUS_HIP_CHECK(hipMalloc(&Y, vec));
+		BACCHUS_HIP_CHECK(hipMalloc(&dC, sizeof(double) * (size_t)(max_blocks + 1) * b * kMaxBlock));
+		BACCHUS_HIP_CHECK(hipMalloc(&dpart, sizeof(double) * (size_t)(max_blocks + 1) * kInnerChunks * b * b));
+		return 0;
+	}
+
+	// C[(j b + p) b + q] = Q_j[:, p] . V[:, q] for the blocks j0 <= j < j0 + nblk
+	int inner(const float *Qs, int nblk, const float *V, std::vector<double> &C)
+	{
+		const int bb = b * b;
+		hipLaunchKernelGGL(block_inner, dim3(kInnerChunks, (unsigned)nblk), dim3(256), 0, nullptr, Qs, V,
+				   D, b, dpart);
+		hpart.resize((size_t)nblk * kInnerChunks * bb);
+		BACCHUS_HIP_CHECK(hipMemcpy(hpart.data(), dpart, hpart.size() * sizeof(double),
+					    hipMemcpyDeviceToHost));
+		C.assign((size_t)nblk * bb, 0.0);
+		for (int j = 0; j < nblk; j++)
+			for (int e = 0; e < bb; e++) {
+				double s = 0;
+				for (int c = 0; c < kInnerChunks; c++)
+					s += hpart[((size_t)j * kInnerChunks + c) * bb + e];
+				C[(size_t)j * bb + e] = s;
+			}
+		return 0;
+	}
+
+	// out = in + Qs C ((nblk b) x m coefficients)
+	template <typename OT>
+	int combine(const float *Qs, int nblk, const std::vector<double> &C, int m, const float *in, OT *out,
+		    int row_major)
+	{
+		BACCHUS_HIP_CHECK(hipMemcpy(dC, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+		hipLaunchKernelGGL(block_combine<OT>, dim3(grid_of(D)), dim3(256), 0, nullptr, Qs, nblk, b, D, dC,
+				   m, in, out, row_major);
+		BACCHUS_HIP_CHECK(hipGetLastError());
+		return 0;
+	}
+
+	// Projects V off the first nblk stored blocks: V <- V - Q C, C added
+	// into coef ((nblk b) x b, may be null)
+	int project_out(int nblk, float *V, std::vector<double> *coef)
+	{
+		if (nblk == 0)
+			return 0;
+		std::vector<double> C;
+		if (int rc = inner(Q, nblk, V, C))
+			return rc;
+		if (coef)
+			for (size_t i = 0; i < C.size(); i++)
+				(*coef)[i] += C[i];
+		for (double &c : C)
+			c = -c;
+		return combine<float>(Q, nblk, C, b, V, V, 0);
+	}
+
+	// V (orthogonal to the stored blocks) = out R, out orthonormal: SVQB
+	// (the Gram matrix's eigensystem); directions whose Gram eigenvalue is
+	// below floor get seeded random columns, orthogonalized, with zero
+	// rows in R.  V, out and Y are distinct buffers.
+	int svqb(int nblk, float *V, float *out, std::vector<double> &R, double floor, uint64_t seed)
+	{
+		std::vector<double> G, lam, U;
+		if (int rc = inner(V, 1, V, G))
+			return rc;
+		jacobi_eigen(b, G, lam, U);
+		// out = V U diag(lam^-1/2); R = diag(lam^1/2) U^T
+		std::vector<double> C((size_t)b * b, 0.0);
+		R.assign((size_t)b * b, 0.0);
+		std::vector<int> dead;
+		for (int k = 0; k < b; k++) {
+			if (!(lam[k] > floor)) {
+				dead.push_back(k);
+				continue;
+			}
+			const double s = std::sqrt(lam[k]);
+			for (int p = 0; p < b; p++) {
+				C[(size_t)p * b + k] = U[(size_t)p * b + k] / s;
+				R[(size_t)k * b + p] = s * U[(size_t)p * b + k];
+			}
+		}
+		if (int rc = combine<float>(V, 1, C, b, nullptr, out, 0))
+			return rc;
+		for (size_t i = 0; i < dead.size(); i++) {
+			// a random direction orthogonal to the stored blocks and to
+			// the block's live columns
+			hipLaunchKernelGGL(block_random_column, dim3(grid_of(D)), dim3(256), 0, nullptr, out,
+					   D, b, dead[i], seed + 7919 * (i + 1));
+		}
+		if (!dead.empty()) {
+			for (int pass = 0; pass < 2; pass++) {
+				if (int rc = project_out(nblk, out, nullptr))
+					return rc;
+				// Gram-Schmidt of the dead columns against the block
+				std::vector<double> Gb;
+				if (int rc = inner(out, 1, out, Gb))
+					return rc;
+				std::vector<double> M((size_t)b * b, 0.0);
+				for (int k = 0; k < b; k++)
+					M[(size_t)k * b + k] = 1.0;
+				std::vector<char> is_dead(b, 0);
+				for (int k : dead)
+					is_dead[k] = 1;
+				for (int k : dead)
+					for (int p = 0; p < b; p++)
+						if (!is_dead[p])
+							M[(size_t)p * b + k] = -Gb[(size_t)p * b + k];
+				if (int rc = combine<float>(out, 1, M, b, nullptr, Y, 0))
+					return rc;
+				BACCHUS_HIP_CHECK(hipMemcpy(out, Y, (size_t)D * b * sizeof(float),
+							    hipMemcpyDeviceToDevice));
+			}
+			// normalize every column once more (dead columns are not
+			// yet orthonormal among themselves); a direction with
+			// nothing left (the Krylov space fills the whole space) stays
+			// zero, which only happens for the block after the last
+			std::vector<double> Gb, l2, U2;
+			if (int rc = inner(out, 1, out, Gb))
+				return rc;
+			jacobi_eigen(b, Gb, l2, U2);
+			const double lmax = *std::max_element(l2.begin(), l2.end());
+			std::vector<double> C2((size_t)b * b, 0.0), R2((size_t)b * b, 0.0);
+			for (int k = 0; k < b; k++) {
+				if (!(l2[k] > 1e-12 * lmax))
+					continue;
+				const double s = std::sqrt(l2[k]);
+				for (int p = 0; p < b; p++) {
+					C2[(size_t)p * b + k] = U2[(size_t)p * b + k] / s;
+					R2[(size_t)k * b + p] = s * U2[(size_t)p * b + k];
+				}
+			}
+			if (int rc = combine<float>(out, 1, C2, b, nullptr, Y, 0))
+				return rc;
+			BACCHUS_HIP_CHECK(hipMemcpy(out, Y, (size_t)D * b * sizeof(float), hipMemcpyDeviceToDevice));
+			// V = out_old R = out_new (R2 R)
+			std::vector<double> RR((size_t)b * b, 0.0);
+			for (int i = 0; i < b; i++)
+				for (int k = 0; k < b; k++)
+					for (int j = 0; j < b; j++)
+						RR[(size_t)i * b + j] += R2[(size_t)i * b + k] * R[(size_t)k * b + j];
+			R.swap(RR);
+		}
+		return 0;
+	}
+
+	// V (b columns) -> out orthonormal and orthogonal to the first nblk
+	// blocks, V = Q coef + out R: two rounds of (project out, SVQB).  V is
+	// overwritten; X serves as the second round's output.
+	int orthonormalize(int nblk, float *V, float *out, std::vector<double> &coef,
+			   std::vector<double> &R, uint64_t seed)
+	{
+		std::vector<double> G0;
+		if (int rc = inner(V, 1, V, G0))
+			return rc;
+		double scale = 0;
+		for (int k = 0; k < b; k++)
+			scale = std::max(scale, G0[(size_t)k * b + k]);
+		const double floor = 1e-24 * scale;
+		coef.assign((size_t)nblk * b * b, 0.0);
+		if (int rc = project_out(nblk, V, &coef))
+			return rc;
+		std::vector<double> R1;
+		if (int rc = svqb(nblk, V, out, R1, floor, seed))
+			return rc;
+		// second round on out: out = Q c2 + out2 R2, so V = Q (coef + c2 R1) + out2 (R2 R1)
+		std::vector<double> c2((size_t)nblk * b * b, 0.0);
+		if (int rc = project_out(nblk, out, &c2))
+			return rc;
+		std::vector<double> R2;
+		if (int rc = svqb(nblk, out, X, R2, 0.0, seed ^ 0x9e3779b9ULL))
+			return rc;
+		BACCHUS_HIP_CHECK(hipMemcpy(out, X, (size_t)D * b * sizeof(float), hipMemcpyDeviceToDevice));
+		for (int i = 0; i < nblk * b; i++)
+			for (int j = 0; j < b; j++) {
+				double s = 0;
+				for (int k = 0; k < b; k++)
+					s += c2[(size_t)i * b + k] * R1[(size_t)k * b + j];
+				coef[(size_t)i * b + j] += s;
+			}
+		R.assign((size_t)b * b, 0.0);
+		for (int i = 0; i < b; i++)
+			for (int k = 0; k < b; k++)
+				for (int j = 0; j < b; j++)
+					R[(size_t)i * b + j] += R2[(size_t)i * b + k] * R1[(size_t)k * b + j];
+		return 0;
+	}
+};
+
+}  // namespace
+}  // namespace bacchus
+
+using namespace bacchus;
+
+extern "C" {
+
+int bacchus_hip_symmetric_eigen(int64_t n, const double *A, double *w, double *V)
+{
+	if (n <= 0 || n > 4096 || !A || !w)
+		return BACCHUS_FAIL("bad arguments (n = %lld)", (long long)n);
+	std::vector<double> M(A, A + n * n), lam, Z;
+	for (int64_t i = 0; i < n; i++)
+		for (int64_t j = 0; j < i; j++) {
+			const double s = 0.5 * (M[i * n + j] + M[j * n + i]);
+			M[i * n + j] = M[j * n + i] = s;
+		}
+	jacobi_eigen((int)n, M, lam, Z);
+	std::copy(lam.begin(), lam.end(), w);
+	if (V)
+		std::copy(Z.begin(), Z.end(), V);
+	return 0;
+}
+
+int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
+			      const bacchus_hip_block_lanczos_settings_t *set, double *eigenvalues,
+			      double *residuals, double *ritz_vectors,
+			      bacchus_hip_lanczos_result_t *result)
+{
+	const auto t_start = std::chrono::steady_clock::now();
+	if (!batch || !set || !eigenvalues)
+		return BACCHUS_FAIL("NULL argument");
+	int32_t num_ops = 0, b = 0;
+	int64_t D = 0;
+	if (int rc = batch_shape(batch, &num_ops, &b, &D))
+		return rc;
+	if (num_ops != 1)
+		return BACCHUS_FAIL("the batch must hold one operator (has %d)", num_ops);
+	if (b < 1 || b > kMaxBlock || b > D)
+		return BACCHUS_FAIL("block size %d outside 1..%d (and the dimension %lld)", b, kMaxBlock,
+				    (long long)D);
+	const int nev = set->num_eigenvalues;
+	if (nev < 1 || nev > b)
+		return BACCHUS_FAIL("num_eigenvalues %d outside 1..%d (the block size)", nev, b);
+	int J = set->max_num_blocks;
+	if (J < 1 || (int64_t)J * b > kMaxKrylov)
+		return BACCHUS_FAIL("max_num_blocks %d: Krylov dimension must stay within %d", J, kMaxKrylov);
+	// every multiplied block is a full set of new directions
+	J = (int)std::min<int64_t>(J, D / b);
+	const int conv = set->convergence;
+	if (conv != BACCHUS_HIP_CONVERGE_TARGETS && conv != BACCHUS_HIP_CONVERGE_RESIDUALS &&
+	    conv != BACCHUS_HIP_CONVERGE_NONE)
+		return BACCHUS_FAIL("convergence %d not supported by the block solver", conv);
+	if (conv == BACCHUS_HIP_CONVERGE_TARGETS && !set->target_eigenvalues)
+		return BACCHUS_FAIL("CONVERGE_TARGETS needs target_eigenvalues");
+
+	Block L;
+	L.D = D;
+	L.b = b;
+	L.max_blocks = J;
+	if (int rc = L.alloc())
+		return rc;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	BACCHUS_HIP_CHECK(hipEventCreate(&e0));
+	BACCHUS_HIP_CHECK(hipEventCreate(&e1));
+	struct Events {
+		hipEvent_t a, b;
+		~Events()
+		{
+			(void)hipEventDestroy(a);
+			(void)hipEventDestroy(b);
+		}
+	} ev_guard{e0, e1};
+
+	// start block: given or seeded random, orthonormalized
+	if (set->start_block)
+		BACCHUS_HIP_CHECK(hipMemcpy(L.W, set->start_block, (size_t)D * b * sizeof(float),
+					    hipMemcpyDeviceToDevice));
+	else
+		for (int k = 0; k < b; k++)
+			hipLaunchKernelGGL(block_random_column, dim3(grid_of(D)), dim3(256), 0, nullptr, L.W, D,
+					   b, k, set->seed * 0x100000001b3ULL + (uint64_t)k);
+	std::vector<double> coef, R;
+	if (int rc = L.orthonormalize(0, L.W, L.blk(0), coef, R, set->seed + 1))
+		return rc;
+
+	const int K = J * b;
+	std::vector<double> T((size_t)(K + b) * (K + b), 0.0);
+	std::vector<double> theta, S, res(nev, 0.0);
+	int done = 0;
+	bool converged = false;
+	for (int j = 0; j < J; j++) {
+		// W = H Q_j
+		BACCHUS_HIP_CHECK(hipMemcpy(L.X, L.blk(j), (size_t)D * b * sizeof(float),
+					    hipMemcpyDeviceToDevice));
+		BACCHUS_HIP_CHECK(hipEventRecord(e0, nullptr));
+		const float *xs[1] = {L.X};
+		float *ys[1] = {L.W};
+		if (int rc = bacchus_hip_batch_matvec(batch, xs, ys, nullptr))
+			return rc;
+		BACCHUS_HIP_CHECK(hipEventRecord(e1, nullptr));
+		BACCHUS_HIP_CHECK(hipEventSynchronize(e1));
+		float ms = 0;
+		BACCHUS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+		L.matvec_ms += ms;
+		// W = Q_{0..j} coef + Q_{j+1} R
+		if (int rc = L.orthonormalize(j + 1, L.W, L.blk(j + 1), coef, R,
+					      set->seed + 0x51ed27ULL * (uint64_t)(j + 2)))
+			return rc;
+		const int m = (j + 1) * b, ld = K + b;
+		for (int p = 0; p < b; p++)
+			for (int q = 0; q < b; q++) {
+				// diagonal block: Q_j^T H Q_j, symmetrized
+				const double a = 0.5 * (coef[((size_t)j * b + p) * b + q] +
+							coef[((size_t)j * b + q) * b + p]);
+				T[(size_t)(j * b + p) * ld + j * b + q] = a;
+				// H Q_j = ... + Q_{j+1} R: T[j+1][j] = R, T[j][j+1] = R^T
+				T[(size_t)((j + 1) * b + p) * ld + j * b + q] = R[(size_t)p * b + q];
+				T[(size_t)(j * b + q) * ld + (j + 1) * b + p] = R[(size_t)p * b + q];
+			}
+		std::vector<double> Tm((size_t)m * m);
+		for (int p = 0; p < m; p++)
+			for (int q = 0; q < m; q++)
+				Tm[(size_t)p * m + q] = T[(size_t)p * ld + q];
+		jacobi_eigen(m, Tm, theta, S);
+		// residual of Ritz pair k: |R S[j b .. (j+1) b, k]|
+		for (int k = 0; k < nev && k < m; k++) {
+			double s2 = 0;
+			for (int p = 0; p < b; p++) {
+				double s = 0;
+				for (int q = 0; q < b; q++)
+					s += R[(size_t)p * b + q] * S[(size_t)(j * b + q) * m + k];
+				s2 += s * s;
+			}
+			res[k] = std::sqrt(s2);
+		}
+		done = j + 1;
+		bool ok = true;
+		for (int k = 0; k < nev; k++) {
+			if (conv == BACCHUS_HIP_CONVERGE_TARGETS) {
+				const double t = set->target_eigenvalues[k];
+				ok = ok && std::fabs(theta[k] - t) <= set->tolerance * std::fabs(t);
+			} else if (conv == BACCHUS_HIP_CONVERGE_RESIDUALS) {
+				double gap = INFINITY;
+				for (int i = 0; i < m; i++)
+					if (i != k)
+						gap = std::min(gap, std::fabs(theta[i] - theta[k]));
+				const double bound = std::isfinite(gap) && gap > 0 ? res[k] * res[k] / gap : res[k];
+				ok = ok && bound <= set->tolerance * std::fabs(theta[k]);
+			} else {
+				ok = false;
+			}
+		}
+		if (ok) {
+			converged = true;
+			break;
+		}
+	}
+	const int m = done * b;
+	for (int k = 0; k < nev; k++) {
+		eigenvalues[k] = theta[k];
+		if (residuals)
+			residuals[k] = res[k];
+	}
+	if (ritz_vectors) {
+		// Ritz vector k = sum_j Q_j S[j b .. (j+1) b, k], written [nev][D]
+		std::vector<double> C((size_t)m * nev);
+		for (int i = 0; i < m; i++)
+			for (int k = 0; k < nev; k++)
+				C[(size_t)i * nev + k] = S[(size_t)i * m + k];
+		if (int rc = L.combine<double>(L.Q, done, C, nev, nullptr, ritz_vectors, 1))
+			return rc;
+		BACCHUS_HIP_CHECK(hipDeviceSynchronize());
+	}
+	(void)converged;
+	if (result) {
+		result->iterations = done;
+		result->matvec_seconds = L.matvec_ms * 1e-3;
+		result->total_seconds =
+			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+	}
+	return 0;
+}
+
+}  // extern "C"

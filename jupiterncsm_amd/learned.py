@@ -117,6 +117,7 @@ class LearnedStart:
     out_scale: float
     train_loss: float
     target: str
+    rank: int = 0               # leading basis rows that are used
 
     def coefficients(self, points):
         import torch
@@ -198,7 +199,68 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
         if want and log:
             log(f"step {k + 1} loss {loss:.3e}")
     torch.cuda.synchronize()
-    return LearnedStart(U, mlp, mean, scale, float(loss), target)
+    return LearnedStart(U, mlp, mean, scale, float(loss), target, rank)
+
+
+@dataclass
+class ProjectedFamily:
+    """H(c1, c2) = H1 + c1 V2 + c2 V3 restricted to the span of the stored
+    basis U (rows, FP32): the three parts are projected once,
+    P_k = U H_k U^T, and for any (c1, c2) the Rayleigh-Ritz problem
+    (P_1 + c1 P_2 + c2 P_3) s = theta N s, N = U U^T, is an n x n FP64
+    eigenproblem on the host.  Because H is linear in (c1, c2) this is
+    exact within the span; its Ritz vectors (eigenvector continuation over
+    the stored solutions) start the block solve."""
+    basis: object               # torch float32 (n, D)
+    rank: int                   # leading rows of basis that are used
+    parts: np.ndarray           # (3, rank, rank)
+    gram: np.ndarray            # (rank, rank)
+
+    @classmethod
+    def build(cls, operator, basis, rank=None):
+        """Projects H1, V2, V3 of operator (any precision; one FP64
+        single-vector matvec per basis row and part) onto basis[:rank]."""
+        import torch
+        n, D = basis.shape
+        rank = n if rank is None else int(rank)
+        if D != operator.dimension:
+            raise bacchus.BacchusError(f"basis has {D} components, the operator {operator.dimension}")
+        U = basis[:rank].contiguous()
+        parts = np.zeros((3, rank, rank))
+        x = torch.empty(D, dtype=torch.float64, device=basis.device)
+        Y = torch.empty((D, rank), dtype=torch.float64, device=basis.device)
+        y = torch.empty(D, dtype=torch.float64, device=basis.device)
+        try:
+            for k, sc in enumerate(((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))):
+                operator.set_scales(*sc)
+                for i in range(rank):
+                    x.copy_(U[i])
+                    operator.matvec(x, y)
+                    Y[:, i].copy_(y)
+                P = bacchus.basis_project(U, Y)
+                parts[k] = 0.5 * (P + P.T)
+        finally:
+            operator.set_scales(1.0, 1.0, 1.0)
+        return cls(basis, rank, parts, bacchus.basis_gram(U))
+
+    def ritz(self, c1, c2, k):
+        """Lowest k Ritz values and coefficient vectors (rank, k) over the
+        basis rows, normalized in the N inner product."""
+        M = self.parts[0] + c1 * self.parts[1] + c2 * self.parts[2]
+        # N = L L^T; eigen of L^-1 M L^-T
+        L = np.linalg.cholesky(self.gram)
+        Li = np.linalg.inv(L)
+        A = Li @ M @ Li.T
+        w, Z = np.linalg.eigh(0.5 * (A + A.T))
+        return w[:k], Li.T @ Z[:, :k]
+
+    def start_block(self, c1, c2, b):
+        """Float32 device block (D, b): the b lowest Ritz vectors of
+        H(c1, c2) in the basis span."""
+        _, S = self.ritz(c1, c2, b)
+        C = np.zeros((b, self.basis.shape[0]))
+        C[:, :self.rank] = S.T
+        return bacchus.basis_transform(self.basis, C).T.contiguous()
 
 
 def solve_from(operator, c, start_vector, *, num_eigenvalues, targets, tolerance=1e-6,

@@ -129,3 +129,37 @@ def test_element_generator_symmetry_and_selection_rules():
             checked += 1
             assert v != 0.0
     assert checked > 10
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 40, 96])
+def test_symmetric_eigen_matches_lapack(n):
+    """The block solver's host Jacobi eigensolver against numpy's LAPACK
+    eigh, plus the closed form of the tridiagonal Toeplitz matrix
+    (a on the diagonal, b beside it: a + 2 b cos(j pi / (n + 1)))."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w, V = bacchus.symmetric_eigen(A)
+    ref = np.linalg.eigvalsh(A)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(w - ref).max() <= 1e-13 * scale
+    assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-13
+    assert np.abs(A @ V - V * w).max() <= 1e-12 * scale
+    a, b = 2.5, -0.75
+    T = np.diag(np.full(n, a)) + np.diag(np.full(n - 1, b), 1) + np.diag(np.full(n - 1, b), -1)
+    wt, _ = bacchus.symmetric_eigen(T)
+    closed = np.sort(a + 2 * b * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
+    assert np.abs(wt - closed).max() <= 1e-13
+
+
+def test_symmetric_eigen_orders_and_symmetrizes():
+    """Ascending order with repeated eigenvalues, and only the symmetric
+    part of a non-symmetric input is used."""
+    A = np.diag([3.0, 1.0, 1.0, -2.0])
+    A[0, 1] = 1e-3            # the solver uses (A + A^T) / 2
+    w, V = bacchus.symmetric_eigen(A)
+    ref = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - ref).max() <= 1e-14
+    with pytest.raises(bacchus.BacchusError):
+        bacchus.symmetric_eigen(np.zeros((2, 3)))
