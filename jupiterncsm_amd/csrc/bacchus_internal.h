@@ -60,17 +60,16 @@ struct TileWork {
 	int32_t vi_begin;
 	int32_t vi_end;
 	int32_t shared;    // 1: other tiles cover the same rows, flush
-			   // atomically; 2: direct tile (N tasks only, no LDS
-			   // tile, sums added to y atomically)
+			   // atomically
 	int32_t t_begin;   // tasks [t_begin, t_end)
 	int32_t t_end;
 	int32_t space;     // 0 normal, 1 transposed
 };
 
-// One unit of a tile's work: CSR rows [jb, je) (output columns) of an N
-// instruction over all rows of the tile (r = -1), or row-side entries
-// [jb, je) (rows of the tile, in row order) of a P/NP instruction with the
-// range [e0, e1) of lane-side entries (NP) or columns (P) it covers.
+// One unit of a tile's work: row-side entries [jb, je) (rows of the tile,
+// in row order) of a P/NP virtual instruction with the range [e0, e1) of
+// lane-side entries (NP) or columns (P) it covers.  (N instructions run as
+// P instructions of the transposed space.)
 struct Task {
 	int32_t vi;
 	int32_t r;
@@ -84,15 +83,9 @@ struct Task {
 
 constexpr int kTileThreads = 256;
 constexpr int kTileMaxElems = 2048;  // doubles of LDS per tile (16 KiB)
-constexpr uint32_t kNTaskEntries = 2048;  // list entries per N task
 constexpr uint64_t kTaskMults = 65536;     // multiplications per P/NP task
-// P instructions of blocks with fewer neutron states than this run in the
-// transposed space (as N instructions there)
-constexpr int64_t kWideRows = 64;
 // lane-side NP entries held by one lane of the single-vector kernel
 constexpr int kLaneEntries = 4;
-// rows of a direct tile (N instructions, register sums, atomics into y)
-constexpr int64_t kDirectRows = 32;
 // XCDs of the device: workgroup i of a launch runs on XCD i mod kXcds
 constexpr int kXcds = 8;
 

@@ -97,10 +97,9 @@ template <> struct Load<uint16_t> {
 //   NP (matrix_vector_multiplication.c:134-210): slots over lane-side
 //     entries; row-side entries of the tile's rows are broadcast with
 //     readlane; sums go to the LDS tile when the row changes.
-//   P (:79-132): slots over columns, same broadcast.
-//   N (:20-77): one output column (a CSR row) at a time, slots striding its
-//     entries, sums for up to 8 tile rows in registers, reduced over the
-//     slots; direct tiles add them to y atomically.
+//   P (:79-132): slots over columns, same broadcast; for narrow rows,
+//     slots of column groups each walking its own entries.
+//   N (:20-77) runs in the transposed space as P there.
 // N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
 __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
@@ -110,8 +109,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int lq = lbt - 2;       // log2(lanes per slot)
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
-	const bool direct = w.shared == 2;  // N only, sums go straight to y
-	const int twords = direct ? 0 : (nrow * nn) << lbt;
+	const int twords = (nrow * nn) << lbt;
 	float *ys = w.space ? o.yT : o.y;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x)
 		tile[t] = 0.0;
@@ -212,6 +210,53 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						}
 				}
 			}
+		} else if (I.type == 2 && 2 * (T.e1 - T.e0) <= (uint32_t)G) {
+			// narrow rows: slots of C >= (columns) lanes groups, C a power
+			// of two, G / C of them; slot g walks entries g, g + G / C, ...
+			// of [jb, je), so its x loads are one contiguous run of
+			// columns x vectors; sums go to the tile on row changes
+			const uint32_t ncol = T.e1 - T.e0;
+			const int lc = ncol <= 1 ? 0 : 32 - __builtin_clz(ncol - 1);
+			const uint32_t Gs = (uint32_t)(G >> lc);
+			const uint32_t g = (uint32_t)(slot >> lc);
+			const uint32_t c = (uint32_t)slot & ((1u << lc) - 1u);
+			const bool valid = c < ncol && bvalid;
+			const uint32_t nl = T.e0 + (c < ncol ? c : 0u);
+			float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+			int cur = -1;
+			for (uint32_t j0 = T.jb + g; j0 < T.je; j0 += 2u * Gs) {
+				uint2 en[2];
+				int rw[2];
+				float e[2];
+				float4 xv[2];
+#pragma unroll
+				for (int u = 0; u < 2; u++) {
+					const uint32_t j = j0 + (uint32_t)u * Gs;
+					const bool ok = j < T.je;
+					en[u] = o.ent[ok ? j : T.jb];
+					rw[u] = ok ? (int)o.keys[j] : -1;
+				}
+#pragma unroll
+				for (int u = 0; u < 2; u++) {
+					e[u] = Load<VT>::at(V, (size_t)I.v_off + (en[u].y & kMask));
+					xv[u] = ld4(xb + ((size_t)nn_in * en[u].x + nl) * B);
+				}
+#pragma unroll
+				for (int u = 0; u < 2; u++) {
+					if (rw[u] < 0)
+						break;
+					if (rw[u] != cur) {
+						if (cur >= 0 && valid)
+							add4_lds(&tile[(((cur - w.po_begin) * nn + (int)nl) << lbt) + q4], sc, a);
+						a = make_float4(0.f, 0.f, 0.f, 0.f);
+						cur = rw[u];
+					}
+					if (!(fabsf(e[u]) < 1e-12f))
+						fma4(a, sign_of(e[u], en[u].y), xv[u]);
+				}
+			}
+			if (cur >= 0 && valid)
+				add4_lds(&tile[(((cur - w.po_begin) * nn + (int)nl) << lbt) + q4], sc, a);
 		} else if (I.type == 2) {
 			for (int n0 = (int)T.e0; n0 < (int)T.e1; n0 += G) {
 				const int n = n0 + slot;
@@ -241,67 +286,6 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				}
 				if (cur >= 0 && valid)
 					add4_lds(&tile[(((cur - w.po_begin) * nn + n) << lbt) + q4], sc, a0);
-			}
-		} else {
-			// CSR rows [jb, je) (output columns): one column at a time,
-			// slots striding its entries, sums for up to 8 tile rows in
-			// registers, then a reduction over the slots
-			const float *xrow0 = xb + (size_t)nn_in * w.po_begin * B;
-			for (uint32_t col = T.jb; col < T.je; col++) {
-				const uint32_t kb = o.rowptr[I.n_base + col];
-				const uint32_t ke = o.rowptr[I.n_base + col + 1];
-				if (kb == ke)
-					continue;
-				for (int r0 = 0; r0 < nrow; r0 += 8) {
-					float4 acc[8];
-#pragma unroll
-					for (int q = 0; q < 8; q++)
-						acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-					for (uint32_t k = kb + slot; k < ke; k += 2 * G) {
-						uint2 en[2];
-						float v[2];
-#pragma unroll
-						for (int u = 0; u < 2; u++)
-							en[u] = k + G * u < ke ? o.ent[k + G * u] : make_uint2(0u, 0u);
-#pragma unroll
-						for (int u = 0; u < 2; u++) {
-							const float e = Load<VT>::at(V, (size_t)I.v_off + (en[u].y & kMask));
-							v[u] = (k + G * u < ke && !(fabsf(e) < 1e-12f)) ? sign_of(e, en[u].y)
-													 : 0.f;
-						}
-#pragma unroll
-						for (int u = 0; u < 2; u++) {
-							const float *xc = xrow0 + ((size_t)nn_in * r0 + en[u].x) * B;
-#pragma unroll
-							for (int q = 0; q < 8; q++)
-								if (r0 + q < nrow)
-									fma4(acc[q], v[u], ld4(xc + (size_t)nn_in * q * B));
-						}
-					}
-#pragma unroll
-					for (int q = 0; q < 8; q++) {
-						if (r0 + q >= nrow)
-							break;
-						float4 v = acc[q];
-						for (int off = 32; off >= (1 << lq); off >>= 1) {
-							v.x += __shfl_xor(v.x, off, 64);
-							v.y += __shfl_xor(v.y, off, 64);
-							v.z += __shfl_xor(v.z, off, 64);
-							v.w += __shfl_xor(v.w, off, 64);
-						}
-						if (slot == 0 && bvalid) {
-							if (direct) {
-								float *d = ys + ((size_t)w.y_off +
-										 (size_t)(w.po_begin + r0 + q) * nn + col) * B + b;
-								atomicAdd(d + 0, sc.x * v.x);
-								atomicAdd(d + 1, sc.y * v.y);
-								atomicAdd(d + 2, sc.z * v.z);
-								atomicAdd(d + 3, sc.w * v.w);
-							} else
-								add4_lds(&tile[(((r0 + q) * nn + (int)col) << lbt) + q4], sc, v);
-						}
-					}
-				}
 			}
 		}
 	}

@@ -90,12 +90,12 @@ template <> struct Elem<uint16_t> {
 //     tile[p_o][n_o(i)] += s_i sum_{j in Lp(p_o)} s_j W[c_i + |C_n| c_j] x[n_i + Nn_in p_j]
 //     with W the instruction's dense element sub-block and c the compact
 //     element indices (see bacchus_hip_operator_create)
-//   N  (:20-77): 16-lane segments per output column n_o (a CSR row of the
-//     list ordered by output index), lanes striding its entries, sums for
-//     every tile row in registers, then a segment reduction:
-//     tile[p_o][n_o] += sum_{i in Ln(n_o)} s_i V[m_i] x[n_i + Nn_in p_o]
-//   P  (:79-132): lanes over columns n, proton entries broadcast;
-//     tile[p_o][n_o] += sum_{j in Lp(p_o)} s_j V[m_j] x[n_o + Nn_in p_j]
+//   P  (:79-132): tile[p_o][n] += sum_{j in Lp(p_o)} s_j V[m_j] x[n + Nn_in p_j]
+//     over the tile's rows p_o and all columns n: lanes over columns,
+//     entries broadcast (rows of more than 32 columns), or slots of lanes
+//     each walking its own entries (narrow rows).
+//   N  (:20-77) runs in the transposed space, where it has the form of P
+//     with the neutron list over the rows and the protons as columns.
 // N and P skip |V| < 1e-12 as the reference does (:52, :107).
 template <typename VT>
 __global__ __launch_bounds__(kTileThreads) void tile_kernel(
@@ -116,8 +116,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 	double *__restrict__ y = w.space ? y_transposed : y_normal;
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
-	const bool direct = w.shared == 2;  // N only, sums go straight to y
-	const int telems = direct ? 0 : nrow * nn;
+	const int telems = nrow * nn;
 	for (int t = threadIdx.x; t < telems; t += blockDim.x)
 		tile[t] = 0.0;
 	__syncthreads();
@@ -206,6 +205,55 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 							atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 				}
 			}
+		} else if (I.type == 2 && T.e1 - T.e0 <= 32u) {
+			// narrow rows (<= 32 columns): 64 / S entry slots of S lanes,
+			// one column per lane; slot g walks entries g, g + G, ... of
+			// [jb, je) (in row order), so a slot's x loads are one
+			// contiguous row segment and its entry and element loads are
+			// shared by its lanes; sums go to the tile on row changes
+			const auto *xblk = x + I.x_off;
+			const uint32_t nn_in = (uint32_t)I.nn_in;
+			const uint32_t ncol = T.e1 - T.e0;
+			const int ls = ncol <= 1 ? 0 : 32 - __builtin_clz(ncol - 1);
+			const int G = 64 >> ls;
+			const int g = lane >> ls;
+			const uint32_t c = (uint32_t)lane & ((1u << ls) - 1u);
+			const bool valid = c < ncol;
+			const uint32_t nl = T.e0 + (valid ? c : 0u);
+			acc_t a = 0;
+			int cur = -1;
+			for (uint32_t j0 = T.jb + (uint32_t)g; j0 < T.je; j0 += 4u * (uint32_t)G) {
+				uint2 en[4];
+				int rw[4];
+				acc_t e[4], xv[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					const uint32_t j = j0 + (uint32_t)(u * G);
+					const bool ok = j < T.je;
+					en[u] = ent[ok ? j : T.jb];
+					rw[u] = ok ? (int)keys[j] : -1;
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					e[u] = Elem<VT>::load(V, I.v_off + (en[u].y & kIndexMask));
+					xv[u] = acc_t((xblk + (size_t)nn_in * en[u].x)[nl]);
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					if (rw[u] < 0)
+						break;
+					if (rw[u] != cur) {
+						if (cur >= 0 && valid && a != acc_t(0))
+							atomicAdd(&tile[(cur - w.po_begin) * nn + nl], (double)(sc * a));
+						a = 0;
+						cur = rw[u];
+					}
+					if (!(fabs(e[u]) < acc_t(1e-12)))
+						a += apply_sign(e[u], en[u].y) * xv[u];
+				}
+			}
+			if (cur >= 0 && valid && a != acc_t(0))
+				atomicAdd(&tile[(cur - w.po_begin) * nn + nl], (double)(sc * a));
 		} else if (I.type == 2) {
 			// proton entries [jb, je) of the tile's rows: lanes over
 			// neighbouring neutron states n (coalesced x); entries broadcast
@@ -245,66 +293,6 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 				}
 				if (cur >= 0 && valid && (a0 + a1) != acc_t(0))
 					atomicAdd(&tile[(cur - w.po_begin) * nn + n], (double)(sc * (a0 + a1)));
-			}
-		} else {
-			// CSR rows [jb, je) of the list ordered by output column: a
-			// 16-lane segment per column, lanes striding its entries (the
-			// entry and its element are loaded once for every tile row);
-			// up to 8 tile rows per pass in registers
-			const auto *xblk = x + I.x_off + (size_t)I.nn_in * w.po_begin;
-			const uint32_t nn_in = (uint32_t)I.nn_in;
-			const int seg = lane >> 4, sl = lane & 15;
-			for (uint32_t c0 = T.jb; c0 < T.je; c0 += 4) {
-				const uint32_t col = c0 + seg;
-				const bool cv = col < T.je;
-				const uint32_t kb = cv ? rowptr[I.n_base + col] : 0;
-				const uint32_t ke = cv ? rowptr[I.n_base + col + 1] : 0;
-				for (int r0 = 0; r0 < nrow; r0 += 8) {
-					acc_t acc[8];
-#pragma unroll
-					for (int q = 0; q < 8; q++)
-						acc[q] = 0;
-					// four entries per lane in flight: their loads are
-					// issued together before any product needs them
-					for (uint32_t k = kb + sl; k < ke; k += 64) {
-						uint2 en[4];
-						acc_t v[4];
-#pragma unroll
-						for (int u = 0; u < 4; u++)
-							en[u] = k + 16 * u < ke ? ent[k + 16 * u] : make_uint2(0u, 0u);
-#pragma unroll
-						for (int u = 0; u < 4; u++) {
-							const acc_t e = Elem<VT>::load(V, I.v_off + (en[u].y & kIndexMask));
-							v[u] = (k + 16 * u < ke && !(fabs(e) < acc_t(1e-12)))
-								       ? apply_sign(e, en[u].y) : acc_t(0);
-						}
-#pragma unroll
-						for (int u = 0; u < 4; u++) {
-							const auto *xc = xblk + en[u].x + (size_t)nn_in * r0;
-#pragma unroll
-							for (int q = 0; q < 8; q++)
-								if (r0 + q < nrow)
-									acc[q] += v[u] * acc_t(xc[(size_t)nn_in * q]);
-						}
-					}
-#pragma unroll
-					for (int q = 0; q < 8; q++) {
-						if (r0 + q >= nrow)
-							break;
-						acc_t v = acc[q];
-						v += __shfl_xor(v, 8, 64);
-						v += __shfl_xor(v, 4, 64);
-						v += __shfl_xor(v, 2, 64);
-						v += __shfl_xor(v, 1, 64);
-						if (sl == 0 && cv && v != acc_t(0)) {
-							if (direct)
-								atomicAdd(y + w.y_off + (int64_t)(w.po_begin + r0 + q) * nn + col,
-									  (double)(sc * v));
-							else
-								atomicAdd(&tile[(r0 + q) * nn + col], (double)(sc * v));
-						}
-					}
-				}
 			}
 		}
 	}
@@ -396,7 +384,6 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	const int64_t nb = (int64_t)op->block_np.size();
 	out = Plan();
 	std::vector<int> tile_bt;
-	std::vector<Task> direct_tasks;
 	for (int64_t slot = 0; slot < 2 * nb; slot++) {
 		const int32_t first = op->blk_vi[slot];
 		int32_t last = op->blk_vi[slot + 1];
@@ -408,65 +395,6 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 		const int64_t nn = space ? op->block_np[b] : op->block_nn[b];
 		const int64_t np = space ? op->block_nn[b] : op->block_np[b];
 		const int bt = pp.bt.empty() ? 1 : pp.bt[slot];
-		// N instructions (at the end of the slot's range): direct tiles of
-		// up to kDirectRows rows and a range of output columns; their sums
-		// go straight to y with atomics, no LDS tile
-		int32_t nfirst = first;
-		while (nfirst < last && tinstr[nfirst].type != 1)
-			nfirst++;
-		for (int32_t v = nfirst; v < last; v++) {
-			const TInstr &I = tinstr[v];
-			for (int64_t po0 = 0; po0 < np; po0 += kDirectRows) {
-				const int64_t po1 = std::min<int64_t>(np, po0 + kDirectRows);
-				const double R = double(po1 - po0);
-				const uint64_t want = std::max<uint64_t>(
-					64, pp.task_mults / std::max<uint64_t>(1, (uint64_t)R * bt));
-				int64_t c0 = 0;
-				while (c0 < nn) {
-					// columns of one tile: about target multiplications
-					int64_t c1 = c0;
-					double cost = 0;
-					while (c1 < nn && (cost == 0 || cost < pp.target)) {
-						cost += double(rowptr[I.n_base + c1 + 1] - rowptr[I.n_base + c1]) *
-							(R + 2.0) * bt;
-						c1++;
-					}
-					TileWork w{};
-					w.y_off = op->block_off[b];
-					w.nn_out = (int32_t)nn;
-					w.po_begin = (int32_t)po0;
-					w.po_end = (int32_t)po1;
-					w.vi_begin = v;
-					w.vi_end = v + 1;
-					w.shared = 2;
-					w.space = space;
-					w.t_begin = (int32_t)direct_tasks.size();
-					uint32_t r0 = (uint32_t)c0;
-					while (r0 < (uint32_t)c1) {
-						uint32_t r1 = r0 + 1;
-						while (r1 < (uint32_t)c1 &&
-						       rowptr[I.n_base + r1 + 1] - rowptr[I.n_base + r0] <= want)
-							r1++;
-						if (rowptr[I.n_base + r1] > rowptr[I.n_base + r0])
-							direct_tasks.push_back(Task{v, -1, r0, r1, 0, 0, 0, 0});
-						r0 = r1;
-					}
-					w.t_end = (int32_t)direct_tasks.size();
-					if (w.t_end > w.t_begin) {
-						out.tiles.push_back(w);
-						out.cost.push_back(cost);
-						tile_bt.push_back(bt);
-					}
-					c0 = c1;
-				}
-			}
-		}
-		// the direct tiles add into the same rows with atomics, so the LDS
-		// tiles of this block must flush atomically as well
-		const bool has_direct = nfirst < last;
-		last = nfirst;
-		if (first == last)
-			continue;
 		if (nn * bt > pp.tile_elems)
 			return BACCHUS_FAIL("basis block %lld has rows of %lld states; a tile "
 					    "of %d vectors holds at most %lld", (long long)b,
@@ -481,8 +409,6 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 				if (t.type == 3)
 					c = double(t.n_len) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
 					    t.n_len * 0.25;
-				else if (t.type == 1)
-					c = double(rowptr[t.n_base + nn] - rowptr[t.n_base]) + nn * 0.25;
 				else
 					c = double(nn) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
 					    nn * 0.25;
@@ -511,7 +437,7 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 			if (pieces == 1) {
 				w.vi_begin = first;
 				w.vi_end = last;
-				w.shared = has_direct ? 1 : 0;
+				w.shared = 0;
 				out.tiles.push_back(w);
 				out.cost.push_back(c);
 				tile_bt.push_back(bt);
@@ -607,35 +533,10 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	}
 	for (size_t ti = 0; ti < out.tiles.size(); ti++) {
 		TileWork &t = out.tiles[ti];
-		if (t.shared == 2) {
-			const int32_t b0 = t.t_begin, b1 = t.t_end;
-			t.t_begin = (int32_t)out.tasks.size();
-			out.tasks.insert(out.tasks.end(), direct_tasks.begin() + b0,
-					 direct_tasks.begin() + b1);
-			t.t_end = (int32_t)out.tasks.size();
-			continue;
-		}
 		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / out.bt[ti]);
 		t.t_begin = (int32_t)out.tasks.size();
 		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
 			const TInstr &I = tinstr[v];
-			if (I.type == 1) {
-				// runs of CSR rows (output columns) holding about
-				// task_mults / rows entries each
-				const uint64_t want = std::max<uint64_t>(
-					64, task_mults / std::max(1, t.po_end - t.po_begin));
-				uint32_t r0 = 0;
-				while (r0 < (uint32_t)t.nn_out) {
-					uint32_t r1 = r0 + 1;
-					while (r1 < (uint32_t)t.nn_out &&
-					       rowptr[I.n_base + r1 + 1] - rowptr[I.n_base + r0] <= want)
-						r1++;
-					if (rowptr[I.n_base + r1] > rowptr[I.n_base + r0])
-						out.tasks.push_back(Task{v, -1, r0, r1, 0, 0, 0, 0});
-					r0 = r1;
-				}
-				continue;
-			}
 			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
 			if (je <= jb)
 				continue;
@@ -888,8 +789,8 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type == 3)
 			swapped[i] = list_len(I.proton_list) > list_len(I.neutron_list);
-		else if (I.type == 2)
-			swapped[i] = op->block_nn[I.block_out] < kWideRows;
+		else
+			swapped[i] = I.type == 1;
 	}
 	// lane-side lists whose elements are used at most about twice (the
 	// 3-body NP lists): their instructions store the element rows expanded
@@ -1114,8 +1015,11 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 					t.n_len = (uint32_t)lp;
 					t.p_rows = (uint32_t)rows[I.neutron_list];
 				} else {
-					t.type = 1;
-					t.n_base = (uint32_t)rows[I.proton_list];
+					// an N instruction there walks its list over the
+					// rows with the protons as contiguous columns, the
+					// form of a P instruction in the normal space
+					t.type = 2;
+					t.p_rows = (uint32_t)rows[I.neutron_list];
 				}
 				per_block[nb + bout].push_back(t);
 			}
@@ -1129,11 +1033,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	op->blk_vi.assign(2 * nb + 1, 0);
 	for (int64_t b = 0; b < 2 * nb; b++) {
 		auto &vs = per_block[b];
-		// N instructions last (they form direct tiles), the others by
-		// element sub-block
-		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
-			return std::make_pair(u.type == 1, u.v_off) < std::make_pair(w.type == 1, w.v_off);
-		});
+		// by element sub-block
+		std::stable_sort(vs.begin(), vs.end(),
+				 [](const TInstr &u, const TInstr &w) { return u.v_off < w.v_off; });
 		op->h_tinstr.insert(op->h_tinstr.end(), vs.begin(), vs.end());
 		op->blk_vi[b + 1] = (int32_t)op->h_tinstr.size();
 		if (b >= nb && !vs.empty())
