@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only run this many FP64 matvecs (for profilers) and exit")
+    ap.add_argument("--block-only", type=int, default=0,
+                    help="only run this many batched FP32 matvecs of one fast-path block "
+                         "(--block vectors, autotuned) and exit (for profilers)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only copy this many MiB device-to-device 3 times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -153,6 +156,20 @@ def main():
         return
 
     op32 = bacchus.Operator.from_inputs(inputs, bacchus.FP32)
+    if args.block_only:
+        with bacchus.Batch([op32], [args.block]) as blk:
+            blk.set_scales(0, np.ones((args.block, 3)))
+            tuned = blk.autotune(3)
+            xs = [torch.randn(D, args.block, dtype=torch.float32, device="cuda")]
+            ys = [torch.empty_like(xs[0])]
+            e0.record(stream)
+            for _ in range(args.block_only):
+                blk.matvec(xs, ys, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            log(f"block matvec ({args.block} vectors, {tuned}): "
+                f"{e0.elapsed_time(e1) / args.block_only:.3f} ms")
+        return
     for _ in range(3):
         op32.matvec(x, y, stream=stream)
     e0.record(stream)
@@ -249,6 +266,9 @@ def main():
     t_fast = float(np.mean(times))
     errs = rel(last.eigenvalues)
     converged = bool(np.all(errs <= args.tolerance))
+    # |H y - theta y| / |theta| of the fast path's Ritz pairs, from its recurrence
+    residuals = ([float(r / abs(t)) for r, t in zip(last.residuals, last.eigenvalues)]
+                 if fam is not None else None)
     steps_taken = last.blocks if fam is not None else last.iterations
     log(f"fast path: {t_fast:.4f}s, {steps_taken} {'blocks' if fam is not None else 'its'}, "
         f"errors {errs}")
@@ -308,6 +328,7 @@ def main():
         "mlp_start": mlp,
         "eigenvalues_fp64_reference": [float(v) for v in targets],
         "relative_errors_fast": [float(v) for v in errs],
+        "relative_residuals_fast": residuals,
         "relative_errors_fp64": [float(v) for v in rel(r64.eigenvalues)],
         "matvec_fp64_ms": round(mv_ms, 4),
         "matvec_fp32_ms": round(mv32_ms, 4),
