@@ -13,6 +13,8 @@
 // first that a persistent grid walks with a static stride.
 #include "bacchus_internal.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -140,6 +142,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 			const VT *vblk = V + I.v_off;
 			const uint32_t dn = (uint32_t)I.dn;
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += kLaneEntries * G) {
+				// slot chunks that hold any entry (wave-uniform); the
+				// loads of empty chunks are skipped
+				const int nc = (int)min((uint32_t)kLaneEntries, (T.e1 - e0 + G - 1) / (uint32_t)G);
 				bool vl[kLaneEntries];
 				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
 				float sgn[kLaneEntries];
@@ -158,6 +163,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					sgn[c] = sign_of(1.f, en.y);
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 				}
+				auto rows_pass = [&](auto full) {
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
@@ -191,8 +197,13 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						float4 xa[kLaneEntries];
 #pragma unroll
 						for (int c = 0; c < kLaneEntries; c++) {
-							va[c] = Load<VT>::at(vblk, vrow + col[c]);
-							xa[c] = ld4(xb + (xrow + xin[c]) * B);
+							if (decltype(full)::value || c < nc) {
+								va[c] = Load<VT>::at(vblk, vrow + col[c]);
+								xa[c] = ld4(xb + (xrow + xin[c]) * B);
+							} else {
+								va[c] = 0.f;
+								xa[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+							}
 						}
 #pragma unroll
 						for (int c = 0; c < kLaneEntries; c++)
@@ -209,6 +220,11 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 							add4_lds(&tile[((r * nn + (int)outc[c]) << lbt) + q4], s4, acc[c]);
 						}
 				}
+				};
+				if (nc == kLaneEntries)
+					rows_pass(std::true_type{});
+				else
+					rows_pass(std::false_type{});
 			}
 		} else if (I.type == 2 && 2 * (T.e1 - T.e0) <= (uint32_t)G) {
 			// narrow rows: slots of C >= (columns) lanes groups, C a power

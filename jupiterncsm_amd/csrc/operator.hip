@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <type_traits>
 #include <tuple>
 
 namespace bacchus {
@@ -144,6 +145,9 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 			const auto *xblk = x + I.x_off;
 			const uint32_t dn = (uint32_t)I.dn, nn_in = (uint32_t)I.nn_in;
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 64 * kLaneEntries) {
+				// chunks of 64 lane entries that hold any (wave-uniform):
+				// the loads of empty chunks are skipped
+				const int nc = (int)min((uint32_t)kLaneEntries, (T.e1 - e0 + 63u) / 64u);
 				bool vl[kLaneEntries];
 				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
 				acc_t sg[kLaneEntries], acc[kLaneEntries];
@@ -161,6 +165,9 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					sg[c] = apply_sign(sc, en.y);
 					acc[c] = 0;
 				}
+				// all chunks full: no checks in the row loop; otherwise
+				// the empty chunks' loads are skipped (wave-uniform)
+				auto rows_pass = [&](auto full) {
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
@@ -189,8 +196,13 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						acc_t va[kLaneEntries], xa[kLaneEntries];
 #pragma unroll
 						for (int c = 0; c < kLaneEntries; c++) {
-							va[c] = Elem<VT>::load(vrow, col[c]);
-							xa[c] = acc_t(xcol[xin[c]]);
+							if (decltype(full)::value || c < nc) {
+								va[c] = Elem<VT>::load(vrow, col[c]);
+								xa[c] = acc_t(xcol[xin[c]]);
+							} else {
+								va[c] = 0;
+								xa[c] = 0;
+							}
 						}
 #pragma unroll
 						for (int c = 0; c < kLaneEntries; c++)
@@ -204,6 +216,11 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						if (vl[c])
 							atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 				}
+				};
+				if (nc == kLaneEntries)
+					rows_pass(std::true_type{});
+				else
+					rows_pass(std::false_type{});
 			}
 		} else if (I.type == 2 && T.e1 - T.e0 <= 32u) {
 			// narrow rows (<= 32 columns): 64 / S entry slots of S lanes,
