@@ -210,6 +210,7 @@ class Operator:
     def __init__(self, handle: ctypes.c_void_p, precision: int):
         self._h = handle
         self.precision = precision
+        self.scales = (1.0, 1.0, 1.0)
         self.dimension = int(library().bacchus_hip_operator_dimension(handle))
 
     # -- construction ------------------------------------------------------
@@ -266,6 +267,7 @@ class Operator:
     def set_scales(self, s1=1.0, s2=1.0, s3=1.0):
         """Scale the 1-, 2- and 3-body parts: H1*s1 + V2*s2 + V3*s3."""
         _check(library().bacchus_hip_operator_set_scales(self.handle, s1, s2, s3))
+        self.scales = (float(s1), float(s2), float(s3))
 
     def matvec(self, x, y, accumulate=False, stream=None):
         """y = H x (or y += H x) on FP64 torch tensors on the GPU."""

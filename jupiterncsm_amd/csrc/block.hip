@@ -475,7 +475,6 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 	std::vector<double> T((size_t)(K + b) * (K + b), 0.0);
 	std::vector<double> theta, S, res(nev, 0.0);
 	int done = 0;
-	bool converged = false;
 	for (int j = 0; j < J; j++) {
 		// W = H Q_j
 		BACCHUS_HIP_CHECK(hipMemcpy(L.X, L.blk(j), (size_t)D * b * sizeof(float),
@@ -538,10 +537,8 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 				ok = false;
 			}
 		}
-		if (ok) {
-			converged = true;
+		if (ok)
 			break;
-		}
 	}
 	const int m = done * b;
 	for (int k = 0; k < nev; k++) {
@@ -559,7 +556,6 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 			return rc;
 		BACCHUS_HIP_CHECK(hipDeviceSynchronize());
 	}
-	(void)converged;
 	if (result) {
 		result->iterations = done;
 		result->matvec_seconds = L.matvec_ms * 1e-3;

@@ -219,7 +219,8 @@ class ProjectedFamily:
     @classmethod
     def build(cls, operator, basis, rank=None):
         """Projects H1, V2, V3 of operator (any precision; one FP64
-        single-vector matvec per basis row and part) onto basis[:rank]."""
+        single-vector matvec per basis row and part) onto basis[:rank];
+        the operator's scales are restored afterwards."""
         import torch
         n, D = basis.shape
         rank = n if rank is None else int(rank)
@@ -230,6 +231,7 @@ class ProjectedFamily:
         x = torch.empty(D, dtype=torch.float64, device=basis.device)
         Y = torch.empty((D, rank), dtype=torch.float64, device=basis.device)
         y = torch.empty(D, dtype=torch.float64, device=basis.device)
+        previous = operator.scales
         try:
             for k, sc in enumerate(((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))):
                 operator.set_scales(*sc)
@@ -240,7 +242,7 @@ class ProjectedFamily:
                 P = bacchus.basis_project(U, Y)
                 parts[k] = 0.5 * (P + P.T)
         finally:
-            operator.set_scales(1.0, 1.0, 1.0)
+            operator.set_scales(*previous)
         return cls(basis, rank, parts, bacchus.basis_gram(U))
 
     def ritz(self, c1, c2, k):
