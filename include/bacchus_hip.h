@@ -247,7 +247,9 @@ int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *batch,
  * launch per block step; Krylov blocks stored FP32 [D][b], inner products
  * in FP64, classical block Gram-Schmidt against all blocks twice, block
  * tridiagonal eigensystem in FP64 on the host.  Krylov dimension
- * b * max_num_blocks <= 256. */
+ * b * max_num_blocks <= 256.  The batch keeps the solver's device workspace
+ * between calls (freed with the batch); calls on one batch must not
+ * overlap. */
 typedef struct bacchus_hip_block_lanczos_settings {
 	int32_t max_num_blocks;
 	int32_t num_eigenvalues;  /* 1..b                                      */

@@ -109,6 +109,9 @@ struct Plan {
 // Shape of a batch: operators, vectors of operator 0, its dimension.
 int batch_shape(const bacchus_hip_batch_t *batch, int32_t *num_ops, int32_t *num_vectors,
 		int64_t *dimension);
+// Device workspace of at least bytes owned by the batch (kept between calls,
+// freed with the batch).
+int batch_workspace(bacchus_hip_batch_t *batch, size_t bytes, void **ptr);
 
 // Host tridiagonal helpers (diag d, off-diagonal e, order k): lowest
 // `count` eigenvalues by Sturm bisection, and the eigenvector of one

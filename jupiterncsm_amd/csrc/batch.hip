@@ -447,7 +447,12 @@ struct bacchus_hip_batch {
 			(void)hipFree(p);
 		(void)hipFree(d_ops);
 		(void)hipFree(d_next);
+		(void)hipFree(workspace);
 	}
+	// device workspace of the block solver, kept between calls (grown on
+	// demand); calls on one batch must not overlap
+	void *workspace = nullptr;
+	size_t workspace_bytes = 0;
 };
 
 namespace bacchus {
@@ -610,6 +615,19 @@ int use_internal(bacchus_hip_batch_t *bt)
 }
 
 }  // namespace
+
+int batch_workspace(bacchus_hip_batch_t *bt, size_t bytes, void **ptr)
+{
+	if (bytes > bt->workspace_bytes) {
+		(void)hipFree(bt->workspace);
+		bt->workspace = nullptr;
+		bt->workspace_bytes = 0;
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->workspace, bytes));
+		bt->workspace_bytes = bytes;
+	}
+	*ptr = bt->workspace;
+	return 0;
+}
 
 int batch_shape(const bacchus_hip_batch_t *bt, int32_t *num_ops, int32_t *num_vectors,
 		int64_t *dimension)

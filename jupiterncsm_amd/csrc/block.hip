@@ -181,26 +181,26 @@ struct Block {
 	std::vector<double> hpart;
 	double matvec_ms = 0;
 
-	~Block()
-	{
-		(void)hipFree(Q);
-		(void)hipFree(X);
-		(void)hipFree(W);
-		(void)hipFree(Y);
-		(void)hipFree(dC);
-		(void)hipFree(dpart);
-	}
 	float *blk(int j) const { return Q + (int64_t)j * D * b; }
 
-	int alloc()
+	// every buffer carved from the batch's workspace (256-byte aligned)
+	int alloc(bacchus_hip_batch_t *batch)
 	{
-		const size_t vec = (size_t)D * b * sizeof(float);
-		BACCHUS_HIP_CHECK(hipMalloc(&Q, vec * (max_blocks + 1)));
-		BACCHUS_HIP_CHECK(hipMalloc(&X, vec));
-		BACCHUS_HIP_CHECK(hipMalloc(&W, vec));
-		BACCHUS_HIP_CHECK(hipMalloc(&Y, vec));
-		BACCHUS_HIP_CHECK(hipMalloc(&dC, sizeof(double) * (size_t)(max_blocks + 1) * b * kMaxBlock));
-		BACCHUS_HIP_CHECK(hipMalloc(&dpart, sizeof(double) * (size_t)(max_blocks + 1) * kInnerChunks * b * b));
+		auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+		const size_t vec = up((size_t)D * b * sizeof(float));
+		const size_t nq = vec * (max_blocks + 1);
+		const size_t nc = up(sizeof(double) * (size_t)(max_blocks + 1) * b * kMaxBlock);
+		const size_t np = up(sizeof(double) * (size_t)(max_blocks + 1) * kInnerChunks * b * b);
+		void *ws = nullptr;
+		if (int rc = batch_workspace(batch, nq + 3 * vec + nc + np, &ws))
+			return rc;
+		char *p = static_cast<char *>(ws);
+		Q = reinterpret_cast<float *>(p);
+		X = reinterpret_cast<float *>(p + nq);
+		W = reinterpret_cast<float *>(p + nq + vec);
+		Y = reinterpret_cast<float *>(p + nq + 2 * vec);
+		dC = reinterpret_cast<double *>(p + nq + 3 * vec);
+		dpart = reinterpret_cast<double *>(p + nq + 3 * vec + nc);
 		return 0;
 	}
 
@@ -445,7 +445,7 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 	L.D = D;
 	L.b = b;
 	L.max_blocks = J;
-	if (int rc = L.alloc())
+	if (int rc = L.alloc(batch))
 		return rc;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	BACCHUS_HIP_CHECK(hipEventCreate(&e0));
