@@ -20,7 +20,9 @@ iterations with full reorthogonalization) with the fast path:
 
 The plain FP64 GPU path (single-vector Lanczos, random start) is timed to
 the target criterion beside it, and so is the single-vector FP32 Lanczos
-from the MLP-predicted start vector over the same basis.
+from the MLP-predicted start vector over the same basis.  The fast path is
+also run at --heldout further (c1, c2) points, each checked against its own
+FP64 reference (fast_path_heldout in the line).
 
 Other modes (one JSON line each):
   --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
@@ -68,6 +70,9 @@ def parse():
                     help="relative size of the random component mixed into the MLP start")
     ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
     ap.add_argument("--max-blocks", type=int, default=16)
+    ap.add_argument("--heldout", type=int, default=8,
+                    help="further (c1, c2) points the fast path is checked on, each against "
+                         "its own FP64 reference")
     ap.add_argument("--batched", action="store_true", help="batched 64-Hamiltonian sweep")
     ap.add_argument("--learned", action="store_true", help="learned-start measurement")
     ap.add_argument("--instances", type=int, default=64)
@@ -273,6 +278,39 @@ def main():
     log(f"fast path: {t_fast:.4f}s, {steps_taken} {'blocks' if fam is not None else 'its'}, "
         f"errors {errs}")
 
+    # --- the fast path at held-out points of the family ---------------------
+    heldout = None
+    if fam is not None and args.heldout > 0:
+        rows = []
+        for c1, c2 in L.family_points(args.heldout, 7):
+            op64.set_scales(1.0, c1, c2)
+            ref_h = bacchus.diagonalize(op64, args.iterations, num_eigenvalues=args.eigenvalues,
+                                        start=bacchus.START_RANDOM, seed=1,
+                                        reorthogonalization=bacchus.REORTH_FULL,
+                                        convergence=bacchus.CONVERGE_NONE).eigenvalues
+            bt.set_scales(0, np.tile((1.0, c1, c2), (args.block, 1)))
+
+            def solve_at(c1=c1, c2=c2):
+                X = fam.start_block(c1, c2, args.block)
+                return bt.block_lanczos(args.max_blocks, args.eigenvalues, start_block=X,
+                                        convergence=bacchus.CONVERGE_RESIDUALS,
+                                        tolerance=args.tolerance)
+            timed(solve_at)
+            dt, r = timed(solve_at)
+            err = float((np.abs(r.eigenvalues - ref_h) / np.abs(ref_h)).max())
+            rows.append((dt, r.blocks, err))
+            log(f"  held-out ({c1:.4f}, {c2:.4f}): {dt * 1e3:.1f} ms, {r.blocks} blocks, "
+                f"max error {err:.2e}")
+        op64.set_scales(1.0, 1.0, 1.0)
+        bt.set_scales(0, np.ones((args.block, 3)))
+        a = np.array(rows)
+        heldout = {"points": len(rows), "seed": 7, "mean_seconds": round(float(a[:, 0].mean()), 4),
+                   "max_seconds": round(float(a[:, 0].max()), 4),
+                   "max_blocks": int(a[:, 1].max()),
+                   "max_relative_error": float(a[:, 2].max()),
+                   "all_within_tolerance": bool((a[:, 2] <= args.tolerance).all())}
+        log(f"held-out: {heldout}")
+
     # --- CPU oracle on the host cores (bounded sample) -------------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -326,6 +364,7 @@ def main():
         "converged": converged,
         "training_seconds": round(train_s, 2),
         "mlp_start": mlp,
+        "fast_path_heldout": heldout,
         "eigenvalues_fp64_reference": [float(v) for v in targets],
         "relative_errors_fast": [float(v) for v in errs],
         "relative_residuals_fast": residuals,
