@@ -245,7 +245,8 @@ int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *batch,
  * lanczos.c:74-253).  The batch's b vectors form the block (1 <= b <= 16);
  * every vector of the batch must carry the same scales.  One batched
  * launch per block step; Krylov blocks stored FP32 [D][b], inner products
- * in FP64, classical block Gram-Schmidt against all blocks twice, block
+ * in FP64, classical block Gram-Schmidt against all blocks twice (once in
+ * the step that converges), block
  * tridiagonal eigensystem in FP64 on the host.  Krylov dimension
  * b * max_num_blocks <= 256.  The batch keeps the solver's device workspace
  * between calls (freed with the batch); calls on one batch must not

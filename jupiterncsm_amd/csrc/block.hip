@@ -8,9 +8,11 @@
 // accumulates in FP64 and the small dense algebra (block tridiagonal
 // matrix, its eigensystem, Gram matrices) runs in FP64 on the host.
 // Orthogonalization is classical block Gram-Schmidt against every stored
-// block, twice, followed by an orthonormalization of the block through the
-// eigensystem of its Gram matrix (SVQB); columns without a direction of
-// their own are refilled with seeded random vectors.
+// block followed by an orthonormalization of the block through the
+// eigensystem of its Gram matrix (SVQB), done twice for every block the
+// solve goes on with (once for the block of the step that converges, whose
+// successor is never used); columns without a direction of their own are
+// refilled with seeded random vectors.
 #include "bacchus_internal.h"
 
 #include <algorithm>
@@ -257,12 +259,15 @@ struct Block {
 	// (the Gram matrix's eigensystem); directions whose Gram eigenvalue is
 	// below floor get seeded random columns, orthogonalized, with zero
 	// rows in R.  V, out and Y are distinct buffers.
-	int svqb(int nblk, float *V, float *out, std::vector<double> &R, double floor, uint64_t seed)
+	int svqb(int nblk, float *V, float *out, std::vector<double> &R, double floor, uint64_t seed,
+		 double *cond = nullptr)
 	{
 		std::vector<double> G, lam, U;
 		if (int rc = inner(V, 1, V, G))
 			return rc;
 		jacobi_eigen(b, G, lam, U);
+		if (cond)
+			*cond = lam[0] > floor && lam[0] > 0 ? lam[b - 1] / lam[0] : INFINITY;
 		// out = V U diag(lam^-1/2); R = diag(lam^1/2) U^T
 		std::vector<double> C((size_t)b * b, 0.0);
 		R.assign((size_t)b * b, 0.0);
@@ -343,10 +348,11 @@ struct Block {
 	}
 
 	// V (b columns) -> out orthonormal and orthogonal to the first nblk
-	// blocks, V = Q coef + out R: two rounds of (project out, SVQB).  V is
-	// overwritten; X serves as the second round's output.
-	int orthonormalize(int nblk, float *V, float *out, std::vector<double> &coef,
-			   std::vector<double> &R, uint64_t seed)
+	// blocks, V = Q coef + out R, in up to two rounds of (project out,
+	// SVQB).  The first round writes out, coef and R and the condition
+	// number of V's Gram matrix after the projection; V is overwritten.
+	int first_round(int nblk, float *V, float *out, std::vector<double> &coef,
+			std::vector<double> &R, uint64_t seed, double *cond)
 	{
 		std::vector<double> G0;
 		if (int rc = inner(V, 1, V, G0))
@@ -358,10 +364,14 @@ struct Block {
 		coef.assign((size_t)nblk * b * b, 0.0);
 		if (int rc = project_out(nblk, V, &coef))
 			return rc;
-		std::vector<double> R1;
-		if (int rc = svqb(nblk, V, out, R1, floor, seed))
-			return rc;
-		// second round on out: out = Q c2 + out2 R2, so V = Q (coef + c2 R1) + out2 (R2 R1)
+		return svqb(nblk, V, out, R, floor, seed, cond);
+	}
+
+	// The second round on out (X is its scratch output): out = Q c2 +
+	// out2 R2, so V = Q (coef + c2 R) + out2 (R2 R); coef and R updated.
+	int second_round(int nblk, float *out, std::vector<double> &coef, std::vector<double> &R,
+			 uint64_t seed)
+	{
 		std::vector<double> c2((size_t)nblk * b * b, 0.0);
 		if (int rc = project_out(nblk, out, &c2))
 			return rc;
@@ -373,14 +383,15 @@ struct Block {
 			for (int j = 0; j < b; j++) {
 				double s = 0;
 				for (int k = 0; k < b; k++)
-					s += c2[(size_t)i * b + k] * R1[(size_t)k * b + j];
+					s += c2[(size_t)i * b + k] * R[(size_t)k * b + j];
 				coef[(size_t)i * b + j] += s;
 			}
-		R.assign((size_t)b * b, 0.0);
+		std::vector<double> RR((size_t)b * b, 0.0);
 		for (int i = 0; i < b; i++)
 			for (int k = 0; k < b; k++)
 				for (int j = 0; j < b; j++)
-					R[(size_t)i * b + j] += R2[(size_t)i * b + k] * R1[(size_t)k * b + j];
+					RR[(size_t)i * b + j] += R2[(size_t)i * b + k] * R[(size_t)k * b + j];
+		R.swap(RR);
 		return 0;
 	}
 };
@@ -468,8 +479,14 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 			hipLaunchKernelGGL(block_random_column, dim3(grid_of(D)), dim3(256), 0, nullptr, L.W, D,
 					   b, k, set->seed * 0x100000001b3ULL + (uint64_t)k);
 	std::vector<double> coef, R;
-	if (int rc = L.orthonormalize(0, L.W, L.blk(0), coef, R, set->seed + 1))
+	double cond = INFINITY;
+	if (int rc = L.first_round(0, L.W, L.blk(0), coef, R, set->seed + 1, &cond))
 		return rc;
+	// a well-conditioned start block is orthonormal to storage precision
+	// after one round
+	if (!(cond < 4.0))
+		if (int rc = L.second_round(0, L.blk(0), coef, R, set->seed + 1))
+			return rc;
 
 	const int K = J * b;
 	std::vector<double> T((size_t)(K + b) * (K + b), 0.0);
@@ -489,21 +506,26 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 		float ms = 0;
 		BACCHUS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
 		L.matvec_ms += ms;
-		// W = Q_{0..j} coef + Q_{j+1} R
-		if (int rc = L.orthonormalize(j + 1, L.W, L.blk(j + 1), coef, R,
-					      set->seed + 0x51ed27ULL * (uint64_t)(j + 2)))
+		// W = Q_{0..j} coef + Q_{j+1} R: the first round decides the
+		// eigenvalues of this step; the second one only matters for the
+		// blocks after it and runs when the solve goes on
+		const uint64_t seed_j = set->seed + 0x51ed27ULL * (uint64_t)(j + 2);
+		if (int rc = L.first_round(j + 1, L.W, L.blk(j + 1), coef, R, seed_j, nullptr))
 			return rc;
 		const int m = (j + 1) * b, ld = K + b;
-		for (int p = 0; p < b; p++)
-			for (int q = 0; q < b; q++) {
-				// diagonal block: Q_j^T H Q_j, symmetrized
-				const double a = 0.5 * (coef[((size_t)j * b + p) * b + q] +
-							coef[((size_t)j * b + q) * b + p]);
-				T[(size_t)(j * b + p) * ld + j * b + q] = a;
-				// H Q_j = ... + Q_{j+1} R: T[j+1][j] = R, T[j][j+1] = R^T
-				T[(size_t)((j + 1) * b + p) * ld + j * b + q] = R[(size_t)p * b + q];
-				T[(size_t)(j * b + q) * ld + (j + 1) * b + p] = R[(size_t)p * b + q];
-			}
+		auto fill_block = [&]() {
+			for (int p = 0; p < b; p++)
+				for (int q = 0; q < b; q++) {
+					// diagonal block: Q_j^T H Q_j, symmetrized
+					const double a = 0.5 * (coef[((size_t)j * b + p) * b + q] +
+								coef[((size_t)j * b + q) * b + p]);
+					T[(size_t)(j * b + p) * ld + j * b + q] = a;
+					// H Q_j = ... + Q_{j+1} R: T[j+1][j] = R, T[j][j+1] = R^T
+					T[(size_t)((j + 1) * b + p) * ld + j * b + q] = R[(size_t)p * b + q];
+					T[(size_t)(j * b + q) * ld + (j + 1) * b + p] = R[(size_t)p * b + q];
+				}
+		};
+		fill_block();
 		std::vector<double> Tm((size_t)m * m);
 		for (int p = 0; p < m; p++)
 			for (int q = 0; q < m; q++)
@@ -537,8 +559,11 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 				ok = false;
 			}
 		}
-		if (ok)
+		if (ok || j + 1 == J)
 			break;
+		if (int rc = L.second_round(j + 1, L.blk(j + 1), coef, R, seed_j))
+			return rc;
+		fill_block();
 	}
 	const int m = done * b;
 	for (int k = 0; k < nev; k++) {
