@@ -3,7 +3,8 @@
 * ``jupiterncsm_amd/libbacchus_hip.so`` - the gfx950 HIP kernels and the C ABI
   declared in ``include/bacchus_hip.h``;
 * ``jupiterncsm_amd/libncsm_inputs.so`` - the synthetic input generator
-  declared in ``include/ncsm_inputs.h``.
+  declared in ``include/ncsm_inputs.h``;
+* ``tests/c_abi/bacchus_driver`` - a plain C caller of the library (test).
 
 Run ``python -m jupiterncsm_amd.build`` or ``__graft_entry__.build()``.
 """
@@ -61,8 +62,20 @@ def build_hip(verbose=False):
     return out
 
 
+def build_c_driver(verbose=False):
+    """The plain C caller of the library used by the C-ABI test
+    (tests/c_abi/bacchus_driver.c), linked as a C build would link it."""
+    src = os.path.join(ROOT, "tests", "c_abi", "bacchus_driver.c")
+    out = os.path.join(ROOT, "tests", "c_abi", "bacchus_driver")
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
+    _run(["gcc", "-std=c99", "-O2", "-Wall", f"-I{INCLUDE}", src, "-o", out, f"-L{HERE}",
+          "-lbacchus_hip", f"-L{rocm_lib}", f"-Wl,-rpath-link,{rocm_lib}",
+          "-Wl,-rpath,$ORIGIN/../../jupiterncsm_amd", f"-Wl,-rpath,{rocm_lib}"], verbose)
+    return out
+
+
 def build_all(verbose=False):
-    return [build_inputs(verbose), build_hip(verbose)]
+    return [build_inputs(verbose), build_hip(verbose), build_c_driver(verbose)]
 
 
 if __name__ == "__main__":

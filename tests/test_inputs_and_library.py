@@ -163,3 +163,21 @@ def test_symmetric_eigen_orders_and_symmetrizes():
     assert np.abs(w - ref).max() <= 1e-14
     with pytest.raises(bacchus.BacchusError):
         bacchus.symmetric_eigen(np.zeros((2, 3)))
+
+
+def test_c_driver_links_and_reports_errors_without_gpu():
+    """The plain C caller (tests/c_abi/bacchus_driver.c, built with the
+    library) links against libbacchus_hip and the HIP runtime, prints its
+    usage on bad arguments (exit 2) and passes the library's error message
+    through for a missing comb.txt (exit 1) - no GPU call on either path."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "c_abi", "bacchus_driver")
+    assert os.path.exists(exe), "build it with __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    with tempfile.TemporaryDirectory() as d:
+        missing = os.path.join(d, "comb.txt")
+        r = subprocess.run([exe, missing, "2", "2", os.path.join(d, "order"), d, d, d, d, "10"],
+                           capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "Error in" in r.stderr and "comb.txt" in r.stderr
