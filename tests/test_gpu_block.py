@@ -138,3 +138,29 @@ def test_projected_family_start_converges_in_one_block():
     assert r.blocks <= 2
     assert err.max() <= 1e-6
     assert rr.blocks > r.blocks
+
+
+def test_batch_and_block_solver_on_an_empty_operator():
+    """No instructions (H = 0, D = 59): the batched matvec writes zeros, and
+    the block solver meets a block that vanishes after one product - every
+    column is refilled with an orthogonalized random direction - and stops
+    at once with zero eigenvalues and residuals."""
+    print("case block lanczos empty operator", flush=True)
+    torch = torch_dev()
+    inp = build(TINY_3B)
+    with bacchus.Operator.from_arrays(inp.blocks[:, 4], inp.blocks[:, 5], [0], np.zeros(0),
+                                      [0], np.zeros(0), np.zeros(0), np.zeros(0),
+                                      np.zeros((0, 8)), bacchus.FP32) as op, \
+            bacchus.Batch([op], [4]) as bt:
+        bt.set_scales(0, np.ones((4, 3)))
+        x = torch.ones((inp.dimension, 4), dtype=torch.float32, device="cuda")
+        y = torch.full_like(x, 7.0)
+        bt.matvec([x], [y])
+        torch.cuda.synchronize()
+        assert float(y.abs().max()) == 0.0
+        r = bt.block_lanczos(4, 3, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=1e-6, seed=3)
+        torch.cuda.synchronize()
+    print("blocks", r.blocks, r.eigenvalues, r.residuals, flush=True)
+    assert r.blocks == 1
+    assert np.abs(r.eigenvalues).max() <= 1e-12
+    assert np.abs(r.residuals).max() <= 1e-12
