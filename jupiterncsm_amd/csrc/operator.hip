@@ -165,9 +165,12 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					sg[c] = apply_sign(sc, en.y);
 					acc[c] = 0;
 				}
-				// all chunks full: no checks in the row loop; otherwise
-				// the empty chunks' loads are skipped (wave-uniform)
-				auto rows_pass = [&](auto full) {
+				// the row loop is compiled for a fixed count NC of chunks
+				// that hold entries (empty chunks then cost nothing), or
+				// with NC = 0 for a count checked at run time
+				auto rows_pass = [&](auto live) {
+				constexpr int NC = decltype(live)::value;
+				constexpr int NL = NC ? NC : kLaneEntries;
 				int cur = -1;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
@@ -180,12 +183,12 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 							if (cur >= 0) {
 								const int r = cur - w.po_begin;
 #pragma unroll
-								for (int c = 0; c < kLaneEntries; c++)
+								for (int c = 0; c < NL; c++)
 									if (vl[c])
 										atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 							}
 #pragma unroll
-							for (int c = 0; c < kLaneEntries; c++)
+							for (int c = 0; c < NL; c++)
 								acc[c] = 0;
 							cur = row;
 						}
@@ -195,8 +198,8 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						const auto *xcol = xblk + (size_t)(nn_in * px);
 						acc_t va[kLaneEntries], xa[kLaneEntries];
 #pragma unroll
-						for (int c = 0; c < kLaneEntries; c++) {
-							if (decltype(full)::value || c < nc) {
+						for (int c = 0; c < NL; c++) {
+							if (NC || c < nc) {
 								va[c] = Elem<VT>::load(vrow, col[c]);
 								xa[c] = acc_t(xcol[xin[c]]);
 							} else {
@@ -205,22 +208,33 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 							}
 						}
 #pragma unroll
-						for (int c = 0; c < kLaneEntries; c++)
+						for (int c = 0; c < NL; c++)
 							acc[c] += apply_sign(va[c], py) * xa[c];
 					}
 				}
 				if (cur >= 0) {
 					const int r = cur - w.po_begin;
 #pragma unroll
-					for (int c = 0; c < kLaneEntries; c++)
+					for (int c = 0; c < NL; c++)
 						if (vl[c])
 							atomicAdd(&tile[r * nn + outc[c]], (double)(sg[c] * acc[c]));
 				}
 				};
+				static_assert(kLaneEntries == 4, "one row loop per count of live chunks");
 				if (nc == kLaneEntries)
-					rows_pass(std::true_type{});
-				else
-					rows_pass(std::false_type{});
+					rows_pass(std::integral_constant<int, 4>{});
+				else if constexpr (sizeof(VT) == sizeof(double)) {
+					// FP64 elements: a copy per count (measured faster
+					// there; the FP32/BF16 kernels lose with the extra
+					// copies)
+					if (nc == 3)
+						rows_pass(std::integral_constant<int, 3>{});
+					else if (nc == 2)
+						rows_pass(std::integral_constant<int, 2>{});
+					else
+						rows_pass(std::integral_constant<int, 1>{});
+				} else
+					rows_pass(std::integral_constant<int, 0>{});
 			}
 		} else if (I.type == 2 && T.e1 - T.e0 <= 32u) {
 			// narrow rows (<= 32 columns): 64 / S entry slots of S lanes,
