@@ -19,10 +19,14 @@ iterations with full reorthogonalization) with the fast path:
   untimed).
 
 The plain FP64 GPU path (single-vector Lanczos, random start) is timed to
-the target criterion beside it, and so is the single-vector FP32 Lanczos
-from the MLP-predicted start vector over the same basis.  The fast path is
-also run at --heldout further (c1, c2) points, each checked against its own
-FP64 reference (fast_path_heldout in the line).
+the target criterion beside it, and so are two starts the MLPs
+(2-256x3-64, GELU, BF16 weights) predict over the same basis: the
+single-vector FP32 Lanczos from one network's start vector (mlp_start), and
+the same block Lanczos from the block of --block networks, each predicting
+the projection of its own fixed vector onto the band (mlp_block_start; its
+own residual bound, and also the target criterion the FP64 path stops on).
+The fast path and the MLP block start are also run at --heldout further
+(c1, c2) points, each checked against its own FP64 reference.
 
 Other modes (one JSON line each):
   --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
@@ -69,6 +73,8 @@ def parse():
     ap.add_argument("--start-mix", type=float, default=1e-4,
                     help="relative size of the random component mixed into the MLP start")
     ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
+    ap.add_argument("--block-mlp-steps", type=int, default=8000,
+                    help="training steps of each MLP of the learned start block (0: none)")
     ap.add_argument("--max-blocks", type=int, default=16)
     ap.add_argument("--heldout", type=int, default=8,
                     help="further (c1, c2) points the fast path is checked on, each against "
@@ -195,7 +201,7 @@ def main():
     log(f"reference eigenvalues ({ref.iterations} its, {time.time() - t0:.2f}s): {targets}")
 
     # --- learned start: trained on batched solves of the family (untimed) -
-    model = fam = bt = None
+    model = fam = bt = mblock = None
     train_s = 0.0
     if not args.no_learned:
         from jupiterncsm_amd import learned as L
@@ -203,8 +209,13 @@ def main():
         pts = L.family_points(args.train_points, 1)
         sol = L.solve_family(op32, pts, iterations=args.train_iterations,
                              num_eigenvalues=args.eigenvalues, target="band",
-                             band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues))
+                             band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues),
+                             band_references=args.block if args.block_mlp_steps else 0)
         model = L.fit(sol, basis_size=64, steps=3000, target="band")
+        if args.block_mlp_steps:
+            mblock = L.fit_block(sol, basis_size=64, steps=args.block_mlp_steps)
+            log(f"MLP start block: {args.block} networks, losses "
+                f"{min(mblock.train_losses):.2e}..{max(mblock.train_losses):.2e}")
         t1 = time.time()
         fam = L.ProjectedFamily.build(op64, model.basis, model.rank)
         train_s = time.time() - t0
@@ -237,11 +248,27 @@ def main():
         sv = model.start_vector(1.0, 1.0, mix=args.start_mix)
         return single(op32, 2, sv)
 
-    def block_solve():
-        X = fam.start_block(1.0, 1.0, args.block)
+    def block_solve(c1=1.0, c2=1.0, start=None, targets_at=None):
+        X = (start or fam).start_block(c1, c2, args.block)
+        if targets_at is not None:
+            return bt.block_lanczos(args.max_blocks, args.eigenvalues, start_block=X,
+                                    convergence=bacchus.CONVERGE_TARGETS, targets=targets_at,
+                                    tolerance=args.tolerance)
         return bt.block_lanczos(args.max_blocks, args.eigenvalues, start_block=X,
                                 convergence=bacchus.CONVERGE_RESIDUALS,
                                 tolerance=args.tolerance)
+
+    def mlp_block_runs(c1, c2, ref_ev):
+        """(seconds, blocks, max relative error) of the block solve from the
+        MLP start block: on its residual bound, then on the target criterion."""
+        out = []
+        for tg in (None, ref_ev):
+            f = lambda: block_solve(c1, c2, mblock, tg)  # noqa: E731
+            timed(f)
+            dt, r = timed(f)
+            out.append((dt, r.blocks,
+                        float((np.abs(r.eigenvalues - ref_ev) / np.abs(ref_ev)).max())))
+        return out
 
     def rel(ev):
         return np.abs(np.asarray(ev) - targets) / np.abs(targets)
@@ -260,6 +287,17 @@ def main():
                "relative_errors": [float(v) for v in rel(rm.eigenvalues)]}
         log(f"fp32 single-vector Lanczos from the MLP start: {tm:.3f}s, {rm.iterations} its, "
             f"errors {rel(rm.eigenvalues)}")
+
+    mlp_block = None
+    if mblock is not None:
+        (tr, br, er), (tt, btg, et) = mlp_block_runs(1.0, 1.0, targets)
+        mlp_block = {"seconds": round(tr, 4), "blocks": br, "max_relative_error": er,
+                     "speedup_vs_fp64_gpu": round(t64 / tr, 3),
+                     "target_criterion": {"seconds": round(tt, 4), "blocks": btg,
+                                          "max_relative_error": et,
+                                          "speedup_vs_fp64_gpu": round(t64 / tt, 3)}}
+        log(f"block Lanczos from the MLP start block: {tr:.4f}s, {br} blocks (residual "
+            f"bound), {tt:.4f}s, {btg} blocks (target criterion)")
 
     fast = block_solve if fam is not None else (lambda: single(op32, 2))
     for _ in range(args.warmup):
@@ -298,9 +336,14 @@ def main():
             timed(solve_at)
             dt, r = timed(solve_at)
             err = float((np.abs(r.eigenvalues - ref_h) / np.abs(ref_h)).max())
-            rows.append((dt, r.blocks, err))
+            row = [dt, r.blocks, err]
+            if mblock is not None:
+                for m in mlp_block_runs(c1, c2, ref_h):
+                    row += list(m)
+            rows.append(row)
             log(f"  held-out ({c1:.4f}, {c2:.4f}): {dt * 1e3:.1f} ms, {r.blocks} blocks, "
-                f"max error {err:.2e}")
+                f"max error {err:.2e}" + (f"; MLP block {row[3] * 1e3:.1f} ms, {row[4]} blocks"
+                                          if mblock is not None else ""))
         op64.set_scales(1.0, 1.0, 1.0)
         bt.set_scales(0, np.ones((args.block, 3)))
         a = np.array(rows)
@@ -309,6 +352,15 @@ def main():
                    "max_blocks": int(a[:, 1].max()),
                    "max_relative_error": float(a[:, 2].max()),
                    "all_within_tolerance": bool((a[:, 2] <= args.tolerance).all())}
+        if mblock is not None:
+            heldout["mlp_block_start"] = {
+                "mean_seconds": round(float(a[:, 3].mean()), 4),
+                "max_seconds": round(float(a[:, 3].max()), 4),
+                "max_blocks": int(a[:, 4].max()),
+                "max_relative_error": float(a[:, 5].max()),
+                "target_criterion_mean_seconds": round(float(a[:, 6].mean()), 4),
+                "target_criterion_max_blocks": int(a[:, 7].max()),
+                "target_criterion_max_relative_error": float(a[:, 8].max())}
         log(f"held-out: {heldout}")
 
     # --- CPU oracle on the host cores (bounded sample) -------------------
@@ -364,6 +416,7 @@ def main():
         "converged": converged,
         "training_seconds": round(train_s, 2),
         "mlp_start": mlp,
+        "mlp_block_start": mlp_block,
         "fast_path_heldout": heldout,
         "eigenvalues_fp64_reference": [float(v) for v in targets],
         "relative_errors_fast": [float(v) for v in errs],

@@ -52,17 +52,22 @@ class FamilySolves:
     seconds: float
     ritz: object = None         # torch float32 (keep_ritz * band, D): the
                                 # band's Ritz vectors of the first points
+    band_targets: list = None   # band targets of further fixed vectors,
+                                # one torch float32 (P, D) per vector
 
 
 def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64, target="ground",
-                 band=5, reference=None, seed=1, keep_targets=True, tolerance=0.0, keep_ritz=0):
+                 band=5, reference=None, seed=1, keep_targets=True, tolerance=0.0, keep_ritz=0,
+                 band_references=0):
     """Batched Lanczos over the points with an FP32/BF16 operator.
 
     reference: float64 device vector (D,) that fixes the ground state's
     sign / is projected onto the band.  Default: the ground state of the
     first point (ground target; its overlap with every other ground state of
     a smooth family stays far from zero, so the sign is continuous in c),
-    or a seeded random vector (band target)."""
+    or a seeded random vector (band target).  band_references > 0 also
+    keeps the band targets of that many further seeded random vectors
+    (band_targets): together they span the band at every point."""
     import torch
     D = operator.dimension
     if reference is None and target != "ground":
@@ -72,6 +77,14 @@ def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64
         reference /= reference.norm()
     nritz = 1 if target == "ground" else int(band)
     evs, tg, kept = [], [], []
+    refs = []
+    if band_references > 0:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(int(seed) + 54321)
+        for _ in range(int(band_references)):
+            r = torch.randn(D, dtype=torch.float64, device="cuda", generator=g)
+            refs.append(r / r.norm())
+    band_tg = [[] for _ in refs]
     t0 = time.time()
     total_it = 0
     for start in range(0, len(points), batch):
@@ -101,12 +114,18 @@ def solve_family(operator, points, *, iterations=80, num_eigenvalues=5, batch=64
                 t = torch.einsum("rdb,rb->db", ritz, ov)
             t = t / t.norm(dim=0, keepdim=True)
             tg.append(t.T.to(torch.float32).contiguous())
+        for k, r in enumerate(refs):
+            ov = torch.einsum("rdb,d->rb", ritz, r)
+            t = torch.einsum("rdb,rb->db", ritz, ov)
+            t = t / t.norm(dim=0, keepdim=True)
+            band_tg[k].append(t.T.to(torch.float32).contiguous())
         del ritz
     targets = torch.cat(tg, 0) if keep_targets else None
     ritz_kept = torch.cat(kept, 0) if kept else None
     torch.cuda.synchronize()
     return FamilySolves(np.asarray(points), np.concatenate(evs, 0), targets, total_it,
-                        time.time() - t0, ritz_kept)
+                        time.time() - t0, ritz_kept,
+                        [torch.cat(b, 0) for b in band_tg] if refs else None)
 
 
 @dataclass
@@ -178,6 +197,17 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
     n = U.shape[0]
     if log:
         log(f"basis rank {rank} of {n}")
+    mlp, mean, scale, loss = _train_coefficients(U, T, solves.points, hidden, depth, steps, lr,
+                                                 seed, log)
+    return LearnedStart(U, mlp, mean, scale, loss, target, rank)
+
+
+def _train_coefficients(U, T, points, hidden, depth, steps, lr, seed, log):
+    """An MLP (c1, c2) -> coordinates of the targets T (P, D) over the
+    stored basis U, trained on the device; returns (mlp, mean, scale,
+    final loss) with the outputs standardized as (Z - mean) / scale."""
+    import torch
+    n = U.shape[0]
     Z = []
     for s in range(0, T.shape[0], 64):
         V = T[s:s + 64].to(torch.float64).T.contiguous()       # (D, b)
@@ -185,7 +215,7 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
     Z = np.concatenate(Z, 0)                                     # (P, n)
     mean = Z.mean(0)
     scale = float(np.abs(Z - mean).max()) or 1.0
-    X = (solves.points - 1.0) / 0.2
+    X = (np.asarray(points) - 1.0) / 0.2
     xd = torch.as_tensor(X, dtype=torch.float32, device="cuda").contiguous()
     td = torch.as_tensor((Z - mean) / scale, dtype=torch.float32, device="cuda").contiguous()
     mlp = bacchus.MLP(2, hidden, depth, n, seed=seed)
@@ -199,7 +229,59 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
         if want and log:
             log(f"step {k + 1} loss {loss:.3e}")
     torch.cuda.synchronize()
-    return LearnedStart(U, mlp, mean, scale, float(loss), target, rank)
+    return mlp, mean, scale, float(loss)
+
+
+@dataclass
+class LearnedBlockStart:
+    """One MLP per fixed reference vector, each predicting the coordinates
+    of that vector's band target over the stored basis: at a new (c1, c2)
+    the predictions together span the band and start the block solve."""
+    basis: object               # torch float32 (n, D)
+    rank: int
+    models: list                # (mlp, mean, scale) per reference vector
+    train_losses: list
+
+    def coefficients(self, c1, c2):
+        """(k, n) FP64 host array of predicted coordinates."""
+        import torch
+        x = torch.as_tensor([[(c1 - 1.0) / 0.2, (c2 - 1.0) / 0.2]], dtype=torch.float32,
+                            device="cuda").contiguous()
+        rows = []
+        for mlp, mean, scale in self.models:
+            y = mlp.forward(x)[0].to(torch.float64).cpu().numpy()
+            rows.append(y * scale + mean)
+        return np.stack(rows, 0)
+
+    def start_block(self, c1, c2, b=None):
+        """Float32 device block (D, b): the predicted band targets,
+        combined on the device (b = number of models by default; extra
+        columns repeat the predictions in order)."""
+        C = self.coefficients(c1, c2)
+        b = C.shape[0] if b is None else int(b)
+        C = C[np.arange(b) % C.shape[0]]
+        if not np.all(np.isfinite(C)):
+            return None
+        return bacchus.basis_transform(self.basis, C).T.contiguous()
+
+
+def fit_block(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
+              seed=1, log=None):
+    """Stored basis as in fit(); one MLP per band_targets entry of the
+    solves, each of the shape fit() trains."""
+    import torch
+    if not solves.band_targets:
+        raise ValueError("solve_family(..., band_references=k) keeps the targets this needs")
+    src = solves.ritz if solves.ritz is not None else solves.targets
+    U, rank = orthonormal_basis(src[:basis_size].contiguous())
+    models, losses = [], []
+    for k, T in enumerate(solves.band_targets):
+        mlp, mean, scale, loss = _train_coefficients(U, T, solves.points, hidden, depth, steps,
+                                                     lr, seed + 101 * k, log)
+        models.append((mlp, mean, scale))
+        losses.append(loss)
+    torch.cuda.synchronize()
+    return LearnedBlockStart(U, rank, models, losses)
 
 
 @dataclass
