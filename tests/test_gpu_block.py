@@ -164,3 +164,57 @@ def test_batch_and_block_solver_on_an_empty_operator():
     assert r.blocks == 1
     assert np.abs(r.eigenvalues).max() <= 1e-12
     assert np.abs(r.residuals).max() <= 1e-12
+
+
+def test_block_lanczos_bf16_agrees_with_single_vector_lanczos():
+    """BF16-stored elements (2+2 Nmax 4): the block solve and the
+    single-vector FP64 Lanczos over the same BF16 operator find the same
+    lowest 5 eigenvalues (both sum in FP32, so to 1e-6), and both lie within
+    the BF16 rounding of the FP64 dense ones."""
+    print("case block lanczos bf16", flush=True)
+    torch = torch_dev()
+    inp = build(SYSTEMS["he4_nmax4"])
+    w = np.linalg.eigvalsh(dense_h(inp, (1.0, 1.0, 1.0)))[:5]
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.BF16) as op, \
+            bacchus.Batch([op], [8]) as bt:
+        bt.set_scales(0, np.ones((8, 3)))
+        r = bt.block_lanczos(30, 5, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=1e-9,
+                             seed=6)
+        s = bacchus.diagonalize(op, 400, num_eigenvalues=5, start=bacchus.START_RANDOM, seed=1,
+                                reorthogonalization=bacchus.REORTH_FULL,
+                                convergence=bacchus.CONVERGE_NONE)
+        torch.cuda.synchronize()
+    d = np.abs(r.eigenvalues - s.eigenvalues) / np.abs(s.eigenvalues)
+    print("blocks", r.blocks, "block vs single", d, flush=True)
+    assert r.blocks < 30
+    assert d.max() <= 1e-6
+    assert (np.abs(r.eigenvalues - w) / np.abs(w)).max() <= 1e-2
+
+
+def test_block_lanczos_stops_on_targets():
+    """The target criterion (the plain FP64 path's stopping rule): the solve
+    stops at the first block whose lowest 5 Ritz values are all within the
+    tolerance of the given targets, which is never later than the residual
+    bound at the same tolerance stops."""
+    print("case block lanczos targets", flush=True)
+    torch = torch_dev()
+    inp = build(SYSTEMS["he4_nmax4"])
+    w = np.linalg.eigvalsh(dense_h(inp, (1.0, 1.0, 1.0)))[:5]
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.FP32) as op, \
+            bacchus.Batch([op], [8]) as bt:
+        bt.set_scales(0, np.ones((8, 3)))
+        rt = bt.block_lanczos(30, 5, convergence=bacchus.CONVERGE_TARGETS, targets=w,
+                              tolerance=1e-6, seed=4)
+        rr = bt.block_lanczos(30, 5, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=1e-6,
+                              seed=4)
+        # one block fewer than it took: not within the tolerance (up to the
+        # FP32 rounding by which two runs of the batched product differ)
+        short = (bt.block_lanczos(rt.blocks - 1, 5, convergence=bacchus.CONVERGE_NONE, seed=4)
+                 if rt.blocks > 1 else None)
+        torch.cuda.synchronize()
+    err = np.abs(rt.eigenvalues - w) / np.abs(w)
+    print("blocks targets", rt.blocks, "residuals", rr.blocks, err, flush=True)
+    assert err.max() <= 1e-6
+    assert rt.blocks <= rr.blocks < 30
+    if short is not None:
+        assert (np.abs(short.eigenvalues - w) / np.abs(w)).max() > 5e-7
