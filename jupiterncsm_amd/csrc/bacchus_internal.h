@@ -42,7 +42,8 @@ struct TInstr {
 	int16_t type;      // 1 N, 2 P, 3 NP
 	int16_t force;     // 1, 2, 3 body
 	int32_t expanded;  // NP: 1 = element rows expanded into lane-list
-			   // order (column = entry position, row stride n_len)
+			   // order (column = entry position, row stride n_len);
+			   // 2 = the same with lanes paired by input (FP64)
 };
 
 // One output tile: rows [po_begin, po_end) of a basis block (all columns)

@@ -60,10 +60,18 @@ __device__ __forceinline__ float apply_sign(float v, uint32_t m)
 }
 
 template <typename VT> struct Elem;
+// load: one element; load2 (FP64): elements i and i + 1 (i even) in one
+// 16-byte load
 template <> struct Elem<double> {
 	using acc_t = double;
 	using x_t = double;
 	__device__ static double load(const double *p, int64_t i) { return p[i]; }
+	__device__ static void load2(const double *p, int64_t i, double &a, double &b)
+	{
+		const double2 v = *reinterpret_cast<const double2 *>(p + i);
+		a = v.x;
+		b = v.y;
+	}
 };
 template <> struct Elem<float> {
 	using acc_t = float;
@@ -132,7 +140,99 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 		const Task T = tasks[ti];
 		const TInstr I = ins[T.vi];
 		const acc_t sc = acc_t(I.force == 1 ? s1 : (I.force == 2 ? s2 : s3));
-		if (I.type == 3) {
+		if (sizeof(VT) == sizeof(double) && I.type == 3 && I.expanded == 2) {
+			if constexpr (sizeof(VT) == sizeof(double)) {
+			// expanded element rows with paired lanes: lane entries 2k and
+			// 2k + 1 share their input index (runs padded with zero
+			// elements), so every lane takes pairs: one 2-element load of
+			// the row and one load of x serve two products.  Otherwise as
+			// below.
+			const VT *vblk = V + I.v_off;
+			const auto *xblk = x + I.x_off;
+			const uint32_t dn = (uint32_t)I.dn, nn_in = (uint32_t)I.nn_in;
+			constexpr int KP = kLaneEntries / 2;  // pairs per lane
+			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 128 * KP) {
+				const int nc = (int)min((uint32_t)KP, (T.e1 - e0 + 127u) / 128u);
+				bool vl[KP];
+				uint32_t col[KP], xin[KP], outa[KP], outb[KP];
+				acc_t sga[KP], sgb[KP], acca[KP], accb[KP];
+#pragma unroll
+				for (int c = 0; c < KP; c++) {
+					// e0, e1 and n_len are even: a pair never straddles
+					// the task's end
+					const uint32_t e = e0 + 128 * c + 2 * lane;
+					vl[c] = e < T.e1;
+					const uint32_t k = I.n_base + (vl[c] ? e : T.e0);
+					const uint2 ea = ent[k], eb = ent[k + 1];
+					col[c] = k - I.n_base;
+					xin[c] = ea.x;
+					outa[c] = keys[k];
+					outb[c] = keys[k + 1];
+					sga[c] = apply_sign(sc, ea.y);
+					sgb[c] = apply_sign(sc, eb.y);
+					acca[c] = accb[c] = 0;
+				}
+				auto pairs_pass = [&](auto live) {
+				constexpr int NC = decltype(live)::value;
+				int cur = -1;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					const uint2 pl = ent[jl];
+					const uint32_t prow = keys[jl];
+					for (uint32_t t = 0; t < cnt; t++) {
+						const int row = (int)__builtin_amdgcn_readlane(prow, t);
+						if (row != cur) {
+							if (cur >= 0) {
+								const int r = cur - w.po_begin;
+#pragma unroll
+								for (int c = 0; c < NC; c++)
+									if (vl[c]) {
+										atomicAdd(&tile[r * nn + outa[c]], (double)(sga[c] * acca[c]));
+										atomicAdd(&tile[r * nn + outb[c]], (double)(sgb[c] * accb[c]));
+									}
+							}
+#pragma unroll
+							for (int c = 0; c < NC; c++)
+								acca[c] = accb[c] = 0;
+							cur = row;
+						}
+						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+						const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+						const auto *xcol = xblk + (size_t)(nn_in * px);
+						acc_t va[KP], vb[KP], xa[KP];
+#pragma unroll
+						for (int c = 0; c < NC; c++) {
+							Elem<VT>::load2(vrow, col[c], va[c], vb[c]);
+							xa[c] = acc_t(xcol[xin[c]]);
+						}
+#pragma unroll
+						for (int c = 0; c < NC; c++) {
+							const acc_t xs = apply_sign(xa[c], py);
+							acca[c] += va[c] * xs;
+							accb[c] += vb[c] * xs;
+						}
+					}
+				}
+				if (cur >= 0) {
+					const int r = cur - w.po_begin;
+#pragma unroll
+					for (int c = 0; c < NC; c++)
+						if (vl[c]) {
+							atomicAdd(&tile[r * nn + outa[c]], (double)(sga[c] * acca[c]));
+							atomicAdd(&tile[r * nn + outb[c]], (double)(sgb[c] * accb[c]));
+						}
+				}
+				};
+				static_assert(KP == 2, "one pass per count of live pair chunks");
+				if (nc == KP)
+					pairs_pass(std::integral_constant<int, 2>{});
+				else
+					pairs_pass(std::integral_constant<int, 1>{});
+			}
+			}
+		} else if (I.type == 3) {
 			// row-side entries [jb, je) of the tile's rows (contiguous in
 			// the copy ordered by row; keys[] gives each entry's row):
 			// every lane holds kLaneEntries neighbouring lane-side entries
@@ -928,8 +1028,15 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	op->list_entry_count = used_entries;
 	// the lane-side list of every NP instruction (neutron list, or proton
 	// list in the transposed space): a copy ordered by element index alone
-	// (rows kept in keys[])
+	// (rows kept in keys[]), or for expanded lists by (input, element).
+	// FP64 operators pad every input's run of an expanded list to an even
+	// length with an entry whose element is zero, so entries 2k and 2k + 1
+	// share their input and the kernel takes them as a pair (measured
+	// faster in FP64 only; FP32/BF16 lose with pairs)
+	const bool pair_lanes = op->precision == BACCHUS_HIP_FP64;
 	std::vector<int64_t> elem_by_out(nl, -1), elem_by_in(nl, -1);
+	std::vector<int64_t> elen_by_out(nl, 0), elen_by_in(nl, 0);
+	std::vector<char> pad_entry(entries.size(), 0);
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type != 3)
@@ -952,13 +1059,23 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
 					return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
 				});
-			(dir == 0 ? elem_by_out : elem_by_in)[l] = (int64_t)entries.size();
-			for (uint32_t k : perm) {
-				const uint2 e = entries[k];
-				const uint32_t r = keys[k];
+			const int64_t start = (int64_t)entries.size();
+			(dir == 0 ? elem_by_out : elem_by_in)[l] = start;
+			for (size_t q = 0; q < perm.size(); q++) {
+				const uint2 e = entries[perm[q]];
+				const uint32_t r = keys[perm[q]];
 				entries.push_back(e);
 				keys.push_back(r);
+				pad_entry.push_back(0);
+				const bool run_ends = q + 1 == perm.size() || entries[perm[q + 1]].x != e.x;
+				if (pair_lanes && expanded[l] && run_ends &&
+				    ((int64_t)entries.size() - start) % 2) {
+					entries.push_back(make_uint2(e.x, 0u));
+					keys.push_back(r);
+					pad_entry.push_back(1);
+				}
 			}
+			(dir == 0 ? elen_by_out : elen_by_in)[l] = (int64_t)entries.size() - start;
 		}
 		if (entries.size() > 0xFFFFFFFFull)
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
@@ -966,7 +1083,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 
 	// expanded element rows of an NP instruction in one direction: row =
 	// compact index of the row-side list, column = position in the lane
-	// copy of that direction
+	// copy of that direction (zero at the padding entries); with paired
+	// lanes, rows start at even element offsets, so an entry pair is one
+	// aligned 2-element load
 	std::map<std::tuple<int32_t, int32_t, int32_t, int>, int64_t> exp_of;
 	auto expanded_rows = [&](const bacchus_hip_instruction_t &I, int64_t i, int dir) -> int64_t {
 		const auto key = std::make_tuple(I.matrix_block, I.neutron_list, I.proton_list, dir);
@@ -976,16 +1095,22 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		const int32_t l = lane_list(I, i);
 		const int32_t rl = swapped[i] ? I.neutron_list : I.proton_list;
 		const int64_t base = (dir == 0 ? elem_by_out : elem_by_in)[l];
-		const int64_t len = list_len(l);
+		const int64_t len = (dir == 0 ? elen_by_out : elen_by_in)[l];
 		const auto &rows = mvals[rl];
 		const auto &cols = mvals[l];
 		const double *Vb = p->matrix_elements + p->matrix_offsets[I.matrix_block];
 		const int64_t Dn = vdn[I.matrix_block];
+		if (pair_lanes && W.size() % 2)
+			W.push_back(0.0);
 		const int64_t off = (int64_t)W.size();
 		W.resize(off + (int64_t)rows.size() * len);
 		double *dst = W.data() + off;
 		for (size_t r = 0; r < rows.size(); r++)
 			for (int64_t k = 0; k < len; k++) {
+				if (pad_entry[base + k]) {
+					dst[r * len + k] = 0.0;
+					continue;
+				}
 				const uint32_t m = cols[entries[base + k].y & 0x7FFFFFFFu];
 				// element index m_n + Dn m_p
 				dst[r * len + k] = swapped[i] ? Vb[rows[r] + Dn * (int64_t)m]
@@ -1019,7 +1144,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			const bool expd = I.type == 3 && expanded[lane_list(I, i)];
 			if (expd) {
 				t.v_off = expanded_rows(I, i, dir);
-				t.expanded = 1;
+				t.expanded = pair_lanes ? 2 : 1;
 			}
 			if (!swapped[i]) {
 				t.dn = I.type == 3 ? (int64_t)mvals[I.neutron_list].size() : 0;
@@ -1027,9 +1152,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				t.type = (int16_t)I.type;
 				if (I.type == 3) {
 					t.n_base = (uint32_t)elem[I.neutron_list];
-					t.n_len = (uint32_t)ln;
+					t.n_len = (uint32_t)(dir == 0 ? elen_by_out : elen_by_in)[I.neutron_list];
 					if (expd)
-						t.dn = (int64_t)ln;
+						t.dn = (int64_t)t.n_len;
 				} else if (I.type == 1)
 					t.n_base = (uint32_t)rows[I.neutron_list];
 				if (I.type != 1)
@@ -1041,9 +1166,9 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				t.nn_in = (int32_t)op->block_np[bin];
 				if (I.type == 3) {
 					t.type = 3;
-					t.dn = expd ? (int64_t)lp : (int64_t)mvals[I.proton_list].size();
 					t.n_base = (uint32_t)elem[I.proton_list];
-					t.n_len = (uint32_t)lp;
+					t.n_len = (uint32_t)(dir == 0 ? elen_by_out : elen_by_in)[I.proton_list];
+					t.dn = expd ? (int64_t)t.n_len : (int64_t)mvals[I.proton_list].size();
 					t.p_rows = (uint32_t)rows[I.neutron_list];
 				} else {
 					// an N instruction there walks its list over the
