@@ -181,3 +181,33 @@ def test_c_driver_links_and_reports_errors_without_gpu():
                            capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "Error in" in r.stderr and "comb.txt" in r.stderr
+
+
+@pytest.mark.parametrize("cfg", [SystemConfig(2, 2, 2, max_body=3, seed=5, sigma3=0.3),
+                                 SystemConfig(3, 3, 2, max_body=3, seed=11, sigma3=0.1)])
+def test_three_body_parity_systems_exercise_paired_lanes(cfg):
+    """The 3-body systems of the FP64 parity tests hold NP lane lists that the
+    FP64 operator stores as expanded element rows (list length at most twice
+    its distinct elements; the lane side is the longer list) with input runs
+    of odd length, so those tests run the paired-lane loop and its zero
+    padding entries, in both directions of off-diagonal instructions."""
+    inp = build(cfg)
+    ins = inp.instructions
+    lo = inp.list_offsets
+    le = inp.list_entries.reshape(-1, 3)
+    lens = np.diff(lo)
+    expanded_np = odd_runs = 0
+    for I in ins[ins["type"] == 3]:
+        ln, lp = lens[I["neutron_list"]], lens[I["proton_list"]]
+        if ln == 0 or lp == 0:
+            continue
+        lane = I["proton_list"] if lp > ln else I["neutron_list"]
+        E = le[lo[lane]:lo[lane + 1]]
+        if len(E) > 2 * len(np.unique(E[:, 2] & 0x7FFFFFFF)):
+            continue
+        expanded_np += 1
+        dirs = (0,) if I["block_in"] == I["block_out"] else (0, 1)
+        for d in dirs:
+            _, runs = np.unique(E[:, d], return_counts=True)
+            odd_runs += int((runs % 2).sum())
+    assert expanded_np > 0 and odd_runs > 0, (expanded_np, odd_runs)
