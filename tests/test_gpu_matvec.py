@@ -140,6 +140,29 @@ def test_c2_size_properties():
     assert abs(x @ hz - z @ hx) <= 1e-12 * np.abs(hx).max() * np.abs(z).sum()
 
 
+def test_c3_full_size_matches_oracle():
+    """The headline workload itself (3+3 Nmax 6, 2+3-body, D = 197,822):
+    the FP64 matvec against the oracle to 1e-12 of max|y| (every kernel
+    path at its real shape: paired and unpaired NP lanes, narrow and wide
+    P rows, both spaces), FP32 storage within 2e-6, and x.Hz = z.Hx."""
+    print("case c3 full size", flush=True)
+    import os
+    inp = build(SYSTEMS["li6_nmax6_3b"])
+    rng = np.random.default_rng(12)
+    x, z = rng.standard_normal(inp.dimension), rng.standard_normal(inp.dimension)
+    ref = O.matvec(O.Problem.from_inputs(inp), x,
+                   threads=min(16, len(os.sched_getaffinity(0))))
+    with bacchus.Operator.from_inputs(inp) as op:
+        hx, hz = gpu_matvec(op, x), gpu_matvec(op, z)
+    print("fp64 rel err", rel_err(hx, ref), flush=True)
+    assert rel_err(hx, ref) <= FP64_TOL
+    assert abs(x @ hz - z @ hx) <= 1e-12 * np.abs(hx).max() * np.abs(z).sum()
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.FP32) as op32:
+        h32 = gpu_matvec(op32, x)
+    print("fp32 rel err", rel_err(h32, ref), flush=True)
+    assert rel_err(h32, ref) <= 2e-6
+
+
 def test_empty_instruction_list_gives_zero():
     print("case empty instructions", flush=True)
     inp = build(CONFIGS["he4_nmax2_3b"])
