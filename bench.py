@@ -26,7 +26,10 @@ the same block Lanczos from the block of --block networks, each predicting
 the projection of its own fixed vector onto the band (mlp_block_start; its
 own residual bound, and also the target criterion the FP64 path stops on).
 The fast path and the MLP block start are also run at --heldout further
-(c1, c2) points, each checked against its own FP64 reference.
+(c1, c2) points, each checked against its own FP64 reference.  The
+FP32- and BF16-stored operators go through the reference's own solve (300
+iterations, same start) as well, and the line states their error per
+eigenvalue (relative_errors_by_storage).
 
 Other modes (one JSON line each):
   --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
@@ -199,6 +202,22 @@ def main():
                               convergence=bacchus.CONVERGE_NONE)
     targets = ref.eigenvalues
     log(f"reference eigenvalues ({ref.iterations} its, {time.time() - t0:.2f}s): {targets}")
+
+    # --- every storage precision through the same solve (300 iterations,
+    # full reorthogonalization, same seeded start, FP64 Lanczos): relative
+    # error of each eigenvalue against the FP64 reference ------------------
+    by_precision = {}
+    for pname, prec in (("fp32", bacchus.FP32), ("bf16", bacchus.BF16)):
+        opx = op32 if prec == bacchus.FP32 else bacchus.Operator.from_inputs(inputs, prec)
+        rx = bacchus.diagonalize(opx, args.iterations, num_eigenvalues=args.eigenvalues,
+                                 start=bacchus.START_RANDOM, seed=1,
+                                 reorthogonalization=bacchus.REORTH_FULL,
+                                 convergence=bacchus.CONVERGE_NONE)
+        if opx is not op32:
+            opx.close()
+        by_precision[pname] = [float(v) for v in np.abs(rx.eigenvalues - targets)
+                               / np.abs(targets)]
+        log(f"{pname} storage: relative eigenvalue errors {by_precision[pname]}")
 
     # --- learned start: trained on batched solves of the family (untimed) -
     model = fam = bt = mblock = None
@@ -422,6 +441,7 @@ def main():
         "relative_errors_fast": [float(v) for v in errs],
         "relative_residuals_fast": residuals,
         "relative_errors_fp64": [float(v) for v in rel(r64.eigenvalues)],
+        "relative_errors_by_storage": by_precision,
         "matvec_fp64_ms": round(mv_ms, 4),
         "matvec_fp32_ms": round(mv32_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
