@@ -15,6 +15,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 #include <tuple>
 
@@ -48,6 +50,47 @@ int set_error(const char *func, const char *file, int line, const char *fmt,
 namespace {
 
 constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
+
+// fn(k) for every k in [0, n) on up to 16 host threads, indices taken from
+// a shared counter; the calling thread works too (and does everything if no
+// thread can be started).  fn must not throw.
+template <typename F> void parallel_for(size_t n, F &&fn)
+{
+	const unsigned hw = std::thread::hardware_concurrency();
+	const size_t nthreads = std::min<size_t>(n, std::max(1u, std::min(16u, hw ? hw : 1u)));
+	std::atomic<size_t> next{0};
+	auto worker = [&]() {
+		for (size_t q; (q = next.fetch_add(1)) < n;)
+			fn(q);
+	};
+	std::vector<std::thread> pool;
+	for (size_t t = 1; t < nthreads; t++) {
+		try {
+			pool.emplace_back(worker);
+		} catch (...) {
+			break;
+		}
+	}
+	worker();
+	for (auto &t : pool)
+		t.join();
+}
+
+// allocator whose resize() leaves new elements uninitialized: every range
+// the element table opens is written in full afterwards, so zeroing it
+// first would only add a pass over gigabytes
+template <typename T> struct NoInitAlloc : std::allocator<T> {
+	template <typename U> struct rebind {
+		using other = NoInitAlloc<U>;
+	};
+	NoInitAlloc() = default;
+	template <typename U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+	template <typename U> void construct(U *q) noexcept { ::new ((void *)q) U; }
+	template <typename U, typename... A> void construct(U *q, A &&...a)
+	{
+		::new ((void *)q) U(std::forward<A>(a)...);
+	}
+};
 
 __device__ __forceinline__ double apply_sign(double v, uint32_t m)
 {
@@ -940,7 +983,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	};
 	std::map<std::tuple<int32_t, int32_t, int32_t>, int64_t> sub_of;
 	std::vector<int64_t> sub_off(ni, -1);
-	std::vector<double> W;
+	std::vector<double, NoInitAlloc<double>> W;
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type == 3 && expanded[lane_list(I, i)])
@@ -980,12 +1023,21 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	}
 
 	// --- CSR copies of every used list, by output and by input index,
-	// each row ordered by element index; keys[k] = row of entry k ---
+	// each row ordered by element index; keys[k] = row of entry k.  Every
+	// (list, direction) gets its place first; the copies are then built by
+	// host threads into their own ranges ---
 	std::vector<uint32_t> rowptr;
 	std::vector<uint2> entries;
 	std::vector<uint32_t> keys;
 	std::vector<int64_t> rows_by_out(nl, -1), rows_by_in(nl, -1);
-	int64_t used_entries = 0;
+	struct CsrJob {
+		int64_t l;
+		int dir;
+		int64_t rp_off;
+		int64_t base;
+	};
+	std::vector<CsrJob> csr_jobs;
+	int64_t used_entries = 0, rp_total = 0, ent_total = 0;
 	for (int64_t l = 0; l < nl; l++) {
 		if (!use[l].used)
 			continue;
@@ -993,38 +1045,47 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		used_entries += b - a;
 		for (int dir = 0; dir < 2; dir++) {
 			const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
-			const int key = dir == 0 ? 1 : 0;  // sort field: out or in
-			(dir == 0 ? rows_by_out : rows_by_in)[l] = (int64_t)rowptr.size();
-			std::vector<uint32_t> count(nrows + 1, 0);
-			for (int64_t k = a; k < b; k++)
-				count[p->list_entries[3 * k + key] + 1]++;
-			const uint32_t base = (uint32_t)entries.size();
-			for (int64_t r = 0; r < nrows; r++)
-				count[r + 1] += count[r];
-			for (int64_t r = 0; r <= nrows; r++)
-				rowptr.push_back(base + count[r]);
-			entries.resize(entries.size() + (b - a));
-			keys.resize(keys.size() + (b - a));
-			std::vector<uint32_t> fill(count.begin(), count.end() - 1);
-			for (int64_t k = a; k < b; k++) {
-				const int32_t *e3 = &p->list_entries[3 * k];
-				const uint32_t r = (uint32_t)e3[key];
-				uint2 en;
-				en.x = (uint32_t)e3[1 - key];
-				en.y = compact(l, e3[2]);
-				keys[base + fill[r]] = r;
-				entries[base + fill[r]++] = en;
-			}
-			for (int64_t r = 0; r < nrows; r++)
-				std::sort(entries.begin() + base + count[r],
-					  entries.begin() + base + count[r + 1],
-					  [](const uint2 &u, const uint2 &w) {
-						  return (u.y & 0x7FFFFFFFu) < (w.y & 0x7FFFFFFFu);
-					  });
-			if (entries.size() > 0xFFFFFFFFull || rowptr.size() > 0xFFFFFFFFull)
-				return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+			(dir == 0 ? rows_by_out : rows_by_in)[l] = rp_total;
+			csr_jobs.push_back(CsrJob{l, dir, rp_total, ent_total});
+			rp_total += nrows + 1;
+			ent_total += b - a;
 		}
 	}
+	if (ent_total > 0xFFFFFFFFll || rp_total > 0xFFFFFFFFll)
+		return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+	rowptr.resize(rp_total);
+	entries.resize(ent_total);
+	keys.resize(ent_total);
+	parallel_for(csr_jobs.size(), [&](size_t q) {
+		const CsrJob &j = csr_jobs[q];
+		const int64_t l = j.l;
+		const int64_t a = p->list_offsets[l], b = p->list_offsets[l + 1];
+		const int64_t nrows = j.dir == 0 ? use[l].n_out : use[l].n_in;
+		const int key = j.dir == 0 ? 1 : 0;  // sort field: out or in
+		std::vector<uint32_t> count(nrows + 1, 0);
+		for (int64_t k = a; k < b; k++)
+			count[p->list_entries[3 * k + key] + 1]++;
+		const uint32_t base = (uint32_t)j.base;
+		for (int64_t r = 0; r < nrows; r++)
+			count[r + 1] += count[r];
+		for (int64_t r = 0; r <= nrows; r++)
+			rowptr[j.rp_off + r] = base + count[r];
+		std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+		for (int64_t k = a; k < b; k++) {
+			const int32_t *e3 = &p->list_entries[3 * k];
+			const uint32_t r = (uint32_t)e3[key];
+			uint2 en;
+			en.x = (uint32_t)e3[1 - key];
+			en.y = compact(l, e3[2]);
+			keys[base + fill[r]] = r;
+			entries[base + fill[r]++] = en;
+		}
+		for (int64_t r = 0; r < nrows; r++)
+			std::sort(entries.begin() + base + count[r], entries.begin() + base + count[r + 1],
+				  [](const uint2 &u, const uint2 &w) {
+					  return (u.y & 0x7FFFFFFFu) < (w.y & 0x7FFFFFFFu);
+				  });
+	});
 	op->list_entry_count = used_entries;
 	// the lane-side list of every NP instruction (neutron list, or proton
 	// list in the transposed space): a copy ordered by element index alone
@@ -1032,53 +1093,91 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	// FP64 operators pad every input's run of an expanded list to an even
 	// length with an entry whose element is zero, so entries 2k and 2k + 1
 	// share their input and the kernel takes them as a pair (measured
-	// faster in FP64 only; FP32/BF16 lose with pairs)
+	// faster in FP64 only; FP32/BF16 lose with pairs).  The copies are
+	// sorted by host threads and appended in instruction order.
 	const bool pair_lanes = op->precision == BACCHUS_HIP_FP64;
 	std::vector<int64_t> elem_by_out(nl, -1), elem_by_in(nl, -1);
 	std::vector<int64_t> elen_by_out(nl, 0), elen_by_in(nl, 0);
 	std::vector<char> pad_entry(entries.size(), 0);
-	for (int64_t i = 0; i < ni; i++) {
-		const bacchus_hip_instruction_t &I = p->instructions[i];
-		if (I.type != 3)
-			continue;
-		const int64_t l = swapped[i] ? I.proton_list : I.neutron_list;
-		if (elem_by_out[l] >= 0)
-			continue;
-		for (int dir = 0; dir < 2; dir++) {
-			const int64_t r0 = (dir == 0 ? rows_by_out : rows_by_in)[l];
-			const int64_t nrows = dir == 0 ? use[l].n_out : use[l].n_in;
-			const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
-			std::vector<uint32_t> perm(b - a);
-			std::iota(perm.begin(), perm.end(), a);
-			if (expanded[l])
-				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
-					return std::make_pair(entries[u].x, entries[u].y & 0x7FFFFFFFu) <
-					       std::make_pair(entries[w].x, entries[w].y & 0x7FFFFFFFu);
-				});
-			else
-				std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
-					return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
-				});
-			const int64_t start = (int64_t)entries.size();
-			(dir == 0 ? elem_by_out : elem_by_in)[l] = start;
-			for (size_t q = 0; q < perm.size(); q++) {
-				const uint2 e = entries[perm[q]];
-				const uint32_t r = keys[perm[q]];
-				entries.push_back(e);
-				keys.push_back(r);
-				pad_entry.push_back(0);
-				const bool run_ends = q + 1 == perm.size() || entries[perm[q + 1]].x != e.x;
-				if (pair_lanes && expanded[l] && run_ends &&
-				    ((int64_t)entries.size() - start) % 2) {
-					entries.push_back(make_uint2(e.x, 0u));
-					keys.push_back(r);
-					pad_entry.push_back(1);
-				}
+	struct LaneJob {
+		int64_t l;
+		int dir;
+		std::vector<uint2> e;
+		std::vector<uint32_t> k;
+		std::vector<char> pad;
+	};
+	std::vector<LaneJob> lane_jobs;
+	{
+		std::vector<char> seen(nl, 0);
+		for (int64_t i = 0; i < ni; i++) {
+			const bacchus_hip_instruction_t &I = p->instructions[i];
+			if (I.type != 3)
+				continue;
+			const int64_t l = swapped[i] ? I.proton_list : I.neutron_list;
+			if (seen[l])
+				continue;
+			seen[l] = 1;
+			for (int dir = 0; dir < 2; dir++) {
+				lane_jobs.emplace_back();
+				lane_jobs.back().l = l;
+				lane_jobs.back().dir = dir;
 			}
-			(dir == 0 ? elen_by_out : elen_by_in)[l] = (int64_t)entries.size() - start;
 		}
-		if (entries.size() > 0xFFFFFFFFull)
+	}
+	parallel_for(lane_jobs.size(), [&](size_t q) {
+		LaneJob &j = lane_jobs[q];
+		const int64_t l = j.l;
+		const int64_t r0 = (j.dir == 0 ? rows_by_out : rows_by_in)[l];
+		const int64_t nrows = j.dir == 0 ? use[l].n_out : use[l].n_in;
+		const uint32_t a = rowptr[r0], b = rowptr[r0 + nrows];
+		std::vector<uint32_t> perm(b - a);
+		std::iota(perm.begin(), perm.end(), a);
+		if (expanded[l])
+			std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+				return std::make_pair(entries[u].x, entries[u].y & 0x7FFFFFFFu) <
+				       std::make_pair(entries[w].x, entries[w].y & 0x7FFFFFFFu);
+			});
+		else
+			std::stable_sort(perm.begin(), perm.end(), [&](uint32_t u, uint32_t w) {
+				return (entries[u].y & 0x7FFFFFFFu) < (entries[w].y & 0x7FFFFFFFu);
+			});
+		j.e.reserve(perm.size());
+		j.k.reserve(perm.size());
+		j.pad.reserve(perm.size());
+		for (size_t t = 0; t < perm.size(); t++) {
+			const uint2 e = entries[perm[t]];
+			const uint32_t r = keys[perm[t]];
+			j.e.push_back(e);
+			j.k.push_back(r);
+			j.pad.push_back(0);
+			const bool run_ends = t + 1 == perm.size() || entries[perm[t + 1]].x != e.x;
+			if (pair_lanes && expanded[l] && run_ends && j.e.size() % 2) {
+				j.e.push_back(make_uint2(e.x, 0u));
+				j.k.push_back(r);
+				j.pad.push_back(1);
+			}
+		}
+	});
+	{
+		size_t total = entries.size();
+		for (const LaneJob &j : lane_jobs)
+			total += j.e.size();
+		if (total > 0xFFFFFFFFull)
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+		entries.reserve(total);
+		keys.reserve(total);
+		pad_entry.reserve(total);
+		for (LaneJob &j : lane_jobs) {
+			const int64_t start = (int64_t)entries.size();
+			(j.dir == 0 ? elem_by_out : elem_by_in)[j.l] = start;
+			(j.dir == 0 ? elen_by_out : elen_by_in)[j.l] = (int64_t)j.e.size();
+			entries.insert(entries.end(), j.e.begin(), j.e.end());
+			keys.insert(keys.end(), j.k.begin(), j.k.end());
+			pad_entry.insert(pad_entry.end(), j.pad.begin(), j.pad.end());
+			std::vector<uint2>().swap(j.e);
+			std::vector<uint32_t>().swap(j.k);
+			std::vector<char>().swap(j.pad);
+		}
 	}
 
 	// expanded element rows of an NP instruction in one direction: row =
@@ -1087,6 +1186,14 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	// lanes, rows start at even element offsets, so an entry pair is one
 	// aligned 2-element load
 	std::map<std::tuple<int32_t, int32_t, int32_t, int>, int64_t> exp_of;
+	// (space is reserved here, in instruction order; the rows are filled
+	// afterwards, in parallel, see fill_expanded)
+	struct ExpJob {
+		int64_t off;
+		int64_t i;
+		int dir;
+	};
+	std::vector<ExpJob> exp_jobs;
 	auto expanded_rows = [&](const bacchus_hip_instruction_t &I, int64_t i, int dir) -> int64_t {
 		const auto key = std::make_tuple(I.matrix_block, I.neutron_list, I.proton_list, dir);
 		auto it = exp_of.find(key);
@@ -1094,17 +1201,26 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			return it->second;
 		const int32_t l = lane_list(I, i);
 		const int32_t rl = swapped[i] ? I.neutron_list : I.proton_list;
-		const int64_t base = (dir == 0 ? elem_by_out : elem_by_in)[l];
 		const int64_t len = (dir == 0 ? elen_by_out : elen_by_in)[l];
+		if (pair_lanes && W.size() % 2)
+			W.push_back(0.0);
+		const int64_t off = (int64_t)W.size();
+		W.resize(off + (int64_t)mvals[rl].size() * len);
+		exp_jobs.push_back(ExpJob{off, i, dir});
+		exp_of[key] = off;
+		return off;
+	};
+	auto fill_expanded = [&](const ExpJob &job) {
+		const bacchus_hip_instruction_t &I = p->instructions[job.i];
+		const int32_t l = lane_list(I, job.i);
+		const int32_t rl = swapped[job.i] ? I.neutron_list : I.proton_list;
+		const int64_t base = (job.dir == 0 ? elem_by_out : elem_by_in)[l];
+		const int64_t len = (job.dir == 0 ? elen_by_out : elen_by_in)[l];
 		const auto &rows = mvals[rl];
 		const auto &cols = mvals[l];
 		const double *Vb = p->matrix_elements + p->matrix_offsets[I.matrix_block];
 		const int64_t Dn = vdn[I.matrix_block];
-		if (pair_lanes && W.size() % 2)
-			W.push_back(0.0);
-		const int64_t off = (int64_t)W.size();
-		W.resize(off + (int64_t)rows.size() * len);
-		double *dst = W.data() + off;
+		double *dst = W.data() + job.off;
 		for (size_t r = 0; r < rows.size(); r++)
 			for (int64_t k = 0; k < len; k++) {
 				if (pad_entry[base + k]) {
@@ -1113,12 +1229,31 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				}
 				const uint32_t m = cols[entries[base + k].y & 0x7FFFFFFFu];
 				// element index m_n + Dn m_p
-				dst[r * len + k] = swapped[i] ? Vb[rows[r] + Dn * (int64_t)m]
-							      : Vb[m + Dn * (int64_t)rows[r]];
+				dst[r * len + k] = swapped[job.i] ? Vb[rows[r] + Dn * (int64_t)m]
+								  : Vb[m + Dn * (int64_t)rows[r]];
 			}
-		exp_of[key] = off;
-		return off;
 	};
+
+	// room for every expanded block at once (no regrowth copies of W)
+	{
+		std::map<std::tuple<int32_t, int32_t, int32_t, int>, int64_t> sizes;
+		for (int64_t i = 0; i < ni; i++) {
+			const bacchus_hip_instruction_t &I = p->instructions[i];
+			if (I.type != 3 || list_len(I.neutron_list) == 0 || list_len(I.proton_list) == 0)
+				continue;
+			const int32_t l = lane_list(I, i);
+			if (!expanded[l])
+				continue;
+			const int32_t rl = swapped[i] ? I.neutron_list : I.proton_list;
+			for (int dir = 0; dir < (I.block_in == I.block_out ? 1 : 2); dir++)
+				sizes[std::make_tuple(I.matrix_block, I.neutron_list, I.proton_list, dir)] =
+					(int64_t)mvals[rl].size() * (dir == 0 ? elen_by_out : elen_by_in)[l] + 1;
+		}
+		int64_t extra = 0;
+		for (const auto &kv : sizes)
+			extra += kv.second;
+		W.reserve(W.size() + (size_t)extra);
+	}
 
 	// --- virtual instructions (each direction) grouped by output block;
 	// slots [0, nb) are the normal space, [nb, 2 nb) the transposed one ---
@@ -1182,6 +1317,24 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		}
 	}
 
+	// expanded element rows: disjoint ranges of W, filled by host threads
+	// (largest first, taken from a shared counter) from read-only tables
+	{
+		std::vector<int64_t> job_size(exp_jobs.size());
+		std::vector<size_t> order(exp_jobs.size());
+		for (size_t k = 0; k < exp_jobs.size(); k++) {
+			const ExpJob &j = exp_jobs[k];
+			const bacchus_hip_instruction_t &I = p->instructions[j.i];
+			const int32_t l = lane_list(I, j.i);
+			const int32_t rl = swapped[j.i] ? I.neutron_list : I.proton_list;
+			job_size[k] = (int64_t)mvals[rl].size() *
+				      (j.dir == 0 ? elen_by_out : elen_by_in)[l];
+			order[k] = k;
+		}
+		std::sort(order.begin(), order.end(),
+			  [&](size_t a, size_t b) { return job_size[a] > job_size[b]; });
+		parallel_for(order.size(), [&](size_t q) { fill_expanded(exp_jobs[order[q]]); });
+	}
 	op->compact_elements = (int64_t)W.size();
 
 	// --- host copies for the planner; virtual instructions of a block
@@ -1265,14 +1418,22 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	if (precision == BACCHUS_HIP_FP64) {
 		rc = upload(&op->d_v, W.data(), total_v * sizeof(double));
 	} else if (precision == BACCHUS_HIP_FP32) {
-		std::vector<float> tmp(total_v);
-		for (int64_t k = 0; k < total_v; k++)
-			tmp[k] = (float)W[k];
+		std::vector<float, NoInitAlloc<float>> tmp(total_v);
+		const int64_t chunk = 1 << 22;
+		parallel_for((size_t)((total_v + chunk - 1) / chunk), [&](size_t c) {
+			const int64_t k1 = std::min(total_v, ((int64_t)c + 1) * chunk);
+			for (int64_t k = (int64_t)c * chunk; k < k1; k++)
+				tmp[k] = (float)W[k];
+		});
 		rc = upload(&op->d_v, tmp.data(), total_v * sizeof(float));
 	} else {
-		std::vector<uint16_t> tmp(total_v);
-		for (int64_t k = 0; k < total_v; k++)
-			tmp[k] = to_bf16(W[k]);
+		std::vector<uint16_t, NoInitAlloc<uint16_t>> tmp(total_v);
+		const int64_t chunk = 1 << 22;
+		parallel_for((size_t)((total_v + chunk - 1) / chunk), [&](size_t c) {
+			const int64_t k1 = std::min(total_v, ((int64_t)c + 1) * chunk);
+			for (int64_t k = (int64_t)c * chunk; k < k1; k++)
+				tmp[k] = to_bf16(W[k]);
+		});
 		rc = upload(&op->d_v, tmp.data(), total_v * sizeof(uint16_t));
 	}
 	if (rc)
