@@ -966,17 +966,25 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		else
 			swapped[i] = I.type == 1;
 	}
-	// lane-side lists whose elements are used at most about twice (the
-	// 3-body NP lists): their instructions store the element rows expanded
-	// into lane-entry order, so lane loads of elements are contiguous and
-	// the lane list can be ordered by input index (clustered x loads)
-	std::vector<char> expanded(nl, 0);
+	// lane-side lists whose elements are used at most about twice, or
+	// three times for 3-body instructions (whose element sub-blocks are too
+	// large to stay in cache, so contiguous rows pay for more copies; a
+	// 2-body sub-block is small and its gathers hit the caches): their
+	// instructions store the element rows expanded into lane-entry order,
+	// so lane loads of elements are contiguous and the lane list can be
+	// ordered by input index (clustered x loads)
+	std::vector<char> expanded(nl, 0), lane_of_3body(nl, 0);
+	for (int64_t i = 0; i < ni; i++) {
+		const bacchus_hip_instruction_t &I = p->instructions[i];
+		if (I.type == 3 && p->matrix_force && p->matrix_force[I.matrix_block] == 3)
+			lane_of_3body[swapped[i] ? I.proton_list : I.neutron_list] = 1;
+	}
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type != 3)
 			continue;
 		const int32_t l = swapped[i] ? I.proton_list : I.neutron_list;
-		expanded[l] = list_len(l) <= 2 * (int64_t)mvals[l].size();
+		expanded[l] = list_len(l) <= (lane_of_3body[l] ? 3 : 2) * (int64_t)mvals[l].size();
 	}
 	auto lane_list = [&](const bacchus_hip_instruction_t &I, int64_t i) -> int32_t {
 		return swapped[i] ? I.proton_list : I.neutron_list;

@@ -188,22 +188,27 @@ def test_c_driver_links_and_reports_errors_without_gpu():
 def test_three_body_parity_systems_exercise_paired_lanes(cfg):
     """The 3-body systems of the FP64 parity tests hold NP lane lists that the
     FP64 operator stores as expanded element rows (list length at most twice
-    its distinct elements; the lane side is the longer list) with input runs
-    of odd length, so those tests run the paired-lane loop and its zero
-    padding entries, in both directions of off-diagonal instructions."""
+    its distinct elements, three times for lane lists of 3-body
+    instructions; the lane side is the longer list) with input runs of odd
+    length, so those tests run the paired-lane loop and its zero padding
+    entries, in both directions of off-diagonal instructions."""
     inp = build(cfg)
     ins = inp.instructions
     lo = inp.list_offsets
     le = inp.list_entries.reshape(-1, 3)
     lens = np.diff(lo)
     expanded_np = odd_runs = 0
-    for I in ins[ins["type"] == 3]:
+    np_ins = [I for I in ins[ins["type"] == 3]
+              if lens[I["neutron_list"]] > 0 and lens[I["proton_list"]] > 0]
+
+    def lane_of(I):
         ln, lp = lens[I["neutron_list"]], lens[I["proton_list"]]
-        if ln == 0 or lp == 0:
-            continue
-        lane = I["proton_list"] if lp > ln else I["neutron_list"]
+        return I["proton_list"] if lp > ln else I["neutron_list"]
+    three_body = {lane_of(I) for I in np_ins if inp.matrix_shapes[I["matrix_block"], 2] == 3}
+    for I in np_ins:
+        lane = lane_of(I)
         E = le[lo[lane]:lo[lane + 1]]
-        if len(E) > 2 * len(np.unique(E[:, 2] & 0x7FFFFFFF)):
+        if len(E) > (3 if lane in three_body else 2) * len(np.unique(E[:, 2] & 0x7FFFFFFF)):
             continue
         expanded_np += 1
         dirs = (0,) if I["block_in"] == I["block_out"] else (0, 1)
