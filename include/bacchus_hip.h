@@ -63,7 +63,10 @@ typedef struct bacchus_hip_problem {
 typedef struct bacchus_hip_operator bacchus_hip_operator_t;
 
 /* Validate the problem on the host (every index in bounds), build the
- * device tables and upload everything to HBM on the current device. */
+ * device tables and upload everything to HBM on the current device.  The
+ * tables are built on up to 16 host threads that end before the call
+ * returns; the problem's arrays are only read.  matrix_force may be NULL
+ * (every block then counts as 2-body for the scales). */
 int bacchus_hip_operator_create(const bacchus_hip_problem_t *problem,
 				int precision, bacchus_hip_operator_t **op);
 
