@@ -31,8 +31,8 @@ def dense_h(inp, scales):
     return 0.5 * (H + H.T)
 
 
-@pytest.mark.parametrize("b,tol", [(8, 1e-10), (5, 1e-8)])
-def test_block_lanczos_random_start_matches_dense(b, tol):
+@pytest.mark.parametrize("b,tol,max_blocks", [(8, 1e-10, 32), (5, 1e-8, 48)])
+def test_block_lanczos_random_start_matches_dense(b, tol, max_blocks):
     """2+2 Nmax 4 (D = 952), H(1, 0.9, 1.1): lowest 5 eigenvalues from a
     random block, stopping on the residual bound, against numpy's dense
     eigh; the Ritz vectors' residuals match the ones the recurrence
@@ -47,13 +47,13 @@ def test_block_lanczos_random_start_matches_dense(b, tol):
             bacchus.Batch([op], [b]) as bt:
         bt.set_scales(0, np.tile(sc, (b, 1)))
         Y = torch.zeros((5, inp.dimension), dtype=torch.float64, device="cuda")
-        r = bt.block_lanczos(30, 5, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=tol,
-                             seed=4, ritz_vectors=Y)
+        r = bt.block_lanczos(max_blocks, 5, convergence=bacchus.CONVERGE_RESIDUALS,
+                             tolerance=tol, seed=4, ritz_vectors=Y)
         torch.cuda.synchronize()
         Yh = Y.cpu().numpy()
     err = np.abs(r.eigenvalues - w) / np.abs(w)
     print("blocks", r.blocks, "errors", err, "residuals", r.residuals, flush=True)
-    assert r.blocks < 30
+    assert r.blocks < max_blocks
     assert err.max() <= 1e-6
     for k in range(5):
         y = Yh[k]
@@ -178,7 +178,7 @@ def test_block_lanczos_bf16_agrees_with_single_vector_lanczos():
     with bacchus.Operator.from_inputs(inp, precision=bacchus.BF16) as op, \
             bacchus.Batch([op], [8]) as bt:
         bt.set_scales(0, np.ones((8, 3)))
-        r = bt.block_lanczos(30, 5, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=1e-9,
+        r = bt.block_lanczos(32, 5, convergence=bacchus.CONVERGE_RESIDUALS, tolerance=1e-9,
                              seed=6)
         s = bacchus.diagonalize(op, 400, num_eigenvalues=5, start=bacchus.START_RANDOM, seed=1,
                                 reorthogonalization=bacchus.REORTH_FULL,
@@ -186,7 +186,7 @@ def test_block_lanczos_bf16_agrees_with_single_vector_lanczos():
         torch.cuda.synchronize()
     d = np.abs(r.eigenvalues - s.eigenvalues) / np.abs(s.eigenvalues)
     print("blocks", r.blocks, "block vs single", d, flush=True)
-    assert r.blocks < 30
+    assert r.blocks < 32
     assert d.max() <= 1e-6
     assert (np.abs(r.eigenvalues - w) / np.abs(w)).max() <= 1e-2
 
