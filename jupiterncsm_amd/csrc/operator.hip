@@ -934,15 +934,16 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	// instruction's element loads are then neighbours and every loaded line
 	// is used. ---
 	std::vector<std::vector<uint32_t>> mvals(nl);
-	for (int64_t l = 0; l < nl; l++) {
+	parallel_for((size_t)nl, [&](size_t l) {
 		if (!use[l].used)
-			continue;
+			return;
 		auto &mv = mvals[l];
+		mv.reserve(p->list_offsets[l + 1] - p->list_offsets[l]);
 		for (int64_t k = p->list_offsets[l]; k < p->list_offsets[l + 1]; k++)
 			mv.push_back((uint32_t)p->list_entries[3 * k + 2] & 0x7FFFFFFFu);
 		std::sort(mv.begin(), mv.end());
 		mv.erase(std::unique(mv.begin(), mv.end()), mv.end());
-	}
+	});
 	auto compact = [&](int64_t l, int32_t m) -> uint32_t {
 		const uint32_t v = (uint32_t)m & 0x7FFFFFFFu;
 		const auto &mv = mvals[l];
@@ -992,6 +993,10 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	std::map<std::tuple<int32_t, int32_t, int32_t>, int64_t> sub_of;
 	std::vector<int64_t> sub_off(ni, -1);
 	std::vector<double, NoInitAlloc<double>> W;
+	// places in instruction order first, then every sub-block filled by
+	// host threads into its own range
+	std::vector<std::pair<int64_t, int64_t>> sub_jobs;  // (offset, instruction)
+	int64_t sub_total = 0;
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type == 3 && expanded[lane_list(I, i)])
@@ -1003,11 +1008,24 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			sub_off[i] = it->second;
 			continue;
 		}
-		const int64_t off = (int64_t)W.size();
+		const int64_t size = I.type == 1 ? (int64_t)mvals[I.neutron_list].size()
+				   : I.type == 2 ? (int64_t)mvals[I.proton_list].size()
+						 : (int64_t)mvals[I.neutron_list].size() *
+							   (int64_t)mvals[I.proton_list].size();
+		sub_jobs.push_back({sub_total, i});
+		sub_of[key] = sub_total;
+		sub_off[i] = sub_total;
+		sub_total += size;
+	}
+	W.resize(sub_total);
+	parallel_for(sub_jobs.size(), [&](size_t q) {
+		const int64_t i = sub_jobs[q].second;
+		const bacchus_hip_instruction_t &I = p->instructions[i];
 		const double *Vb = p->matrix_elements + p->matrix_offsets[I.matrix_block];
+		double *dst = W.data() + sub_jobs[q].first;
 		if (I.type == 1 || I.type == 2) {
 			for (uint32_t m : mvals[I.type == 1 ? I.neutron_list : I.proton_list])
-				W.push_back(Vb[m]);
+				*dst++ = Vb[m];
 		} else {
 			// normal: rows over the proton set, columns over the neutron
 			// set; transposed space: the other way round.  The swap
@@ -1015,8 +1033,6 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 			// instruction sharing the sub-block agrees with it.
 			const auto &mn = mvals[I.neutron_list], &mp = mvals[I.proton_list];
 			const int64_t Dn = vdn[I.matrix_block];
-			W.resize(off + (int64_t)mn.size() * (int64_t)mp.size());
-			double *dst = W.data() + off;
 			for (size_t b = 0; b < mp.size(); b++)
 				for (size_t a2 = 0; a2 < mn.size(); a2++) {
 					const double v = Vb[mn[a2] + Dn * (int64_t)mp[b]];
@@ -1026,9 +1042,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 						dst[b * mn.size() + a2] = v;
 				}
 		}
-		sub_of[key] = off;
-		sub_off[i] = off;
-	}
+	});
 
 	// --- CSR copies of every used list, by output and by input index,
 	// each row ordered by element index; keys[k] = row of entry k.  Every
