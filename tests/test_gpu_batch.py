@@ -203,3 +203,45 @@ def test_batch_rejects_fp64_operator():
     with bacchus.Operator.from_inputs(inp) as op:
         with pytest.raises(bacchus.BacchusError, match="FP64"):
             bacchus.Batch([op], [2])
+
+
+@pytest.mark.parametrize("case", ["N", "P narrow", "P wide", "NP"])
+def test_batched_tiny_element_skip(case):
+    """The batched kernel keeps the reference's 1e-12 element skip for N and P
+    instructions (matrix_vector_multiplication.c:52, :107) and none for NP,
+    on FP32-stored elements: 4 vectors with x of order 1e12, so each skipped
+    product is 0.5 or more, against the oracle in FP64 (FP32 bar 2e-6)."""
+    print("case batched tiny element skip", case, flush=True)
+    torch = torch_dev()
+    # every product of order 1 (no large element hides a wrong one)
+    V = [5e-13, -5e-13, 2e-12, 1e-12]
+    lst = [(0, 1, 0), (1, 0, 1), (2, 2, 2), (0, 0, 3)]
+    if case == "N":
+        np_, nn, lists, ins, dn = 2, 3, [lst], [1, 1, 0, 0, 0, -1, 0, 0], 4
+    elif case.startswith("P"):
+        np_, nn = 3, (5 if case == "P narrow" else 40)
+        lists, ins, dn = [lst], [2, 1, 0, 0, -1, 0, 0, 0], 1
+    else:
+        np_, nn, dn = 2, 3, 2
+        lists = [[(0, 1, 0), (1, 0, 1), (2, 2, 0)], [(0, 0, 0), (1, 1, 0), (0, 1, 1)]]
+        V = [5e-13, -5e-13, 2e-13, 1e-12]
+        ins = [3, 2, 0, 0, 0, 1, 0, 0]
+    offsets = np.cumsum([0] + [len(e) for e in lists])
+    entries = np.array([t for e in lists for t in e], dtype=np.int32).reshape(-1, 3)
+    arrays = ([np_], [nn], offsets, entries, [0, len(V)], [dn], [2], np.array(V), np.array([ins]))
+    D = np_ * nn
+    B = 4
+    X = 1e12 * (1.0 + np.random.default_rng(6).random((D, B)))
+    prob = O.Problem(*arrays)
+    ref = np.stack([O.matvec(prob, X[:, b]) for b in range(B)], axis=1)
+    with bacchus.Operator.from_arrays(*arrays, precision=bacchus.FP32) as op, \
+            bacchus.Batch([op], [B]) as bt:
+        bt.set_scales(0, np.ones((B, 3)))
+        xs = [torch.as_tensor(X, dtype=torch.float32, device="cuda").contiguous()]
+        ys = [torch.zeros_like(xs[0])]
+        bt.matvec(xs, ys)
+        torch.cuda.synchronize()
+        Y = ys[0].cpu().numpy().astype(np.float64)
+    err = np.abs(Y - ref).max() / np.abs(ref).max()
+    print("batched vs oracle", err, flush=True)
+    assert err <= 2e-6

@@ -179,7 +179,8 @@ def test_tiny_element_skip(case):
     product 0.5 or more, far above FP64 rounding: the device result must
     equal the product computed by hand, and the oracle's."""
     print("case tiny element skip", case, flush=True)
-    V = [5e-13, -5e-13, 3.0, 1e-12]
+    # every product of order 1 (no large element hides a wrong one)
+    V = [5e-13, -5e-13, 2e-12, 1e-12]
     lst = [(0, 1, 0), (1, 0, 1), (2, 2, 2), (0, 0, 3)]
     if case == "N":
         np_, nn = 2, 3
