@@ -162,3 +162,44 @@ def test_c1_lowest_eigenvalues_against_dense(c1_inputs):
                        start_vector=np.ones(c1_inputs.dimension) / np.sqrt(c1_inputs.dimension),
                        reorth=O.REORTH_FULL, num_eigenvalues=5)
     assert np.abs(ev - w[:5]).max() < 1e-9 * abs(w[0])
+
+
+@pytest.mark.parametrize("case", ["N", "P", "NP"])
+def test_oracle_tiny_element_skip(case):
+    """The oracle skips |V| < 1e-12 in N and P instructions and keeps exactly
+    1e-12 (matrix_vector_multiplication.c:52, :107); NP skips nothing
+    (:134-210).  Against products written out by hand, with x of order 1e12
+    so every product, kept or skipped, is of order 1."""
+    V = [5e-13, -5e-13, 2e-12, 1e-12]
+    lst = [(0, 1, 0), (1, 0, 1), (2, 2, 2), (0, 0, 3)]
+    if case == "N":
+        np_, nn, lists, ins, dn = 2, 3, [lst], [1, 1, 0, 0, 0, -1, 0, 0], 4
+    elif case == "P":
+        np_, nn, lists, ins, dn = 3, 4, [lst], [2, 1, 0, 0, -1, 0, 0, 0], 1
+    else:
+        np_, nn, dn = 2, 3, 2
+        lists = [[(0, 1, 0), (1, 0, 1), (2, 2, 0)], [(0, 0, 0), (1, 1, 0), (0, 1, 1)]]
+        V = [5e-13, -5e-13, 2e-13, 1e-12]
+        ins = [3, 2, 0, 0, 0, 1, 0, 0]
+    offsets = np.cumsum([0] + [len(e) for e in lists])
+    entries = np.array([t for e in lists for t in e], dtype=np.int32).reshape(-1, 3)
+    prob = O.Problem([np_], [nn], offsets, entries, [0, len(V)], [dn], [2], np.array(V),
+                     np.array([ins]))
+    X = 1e12 * (1.0 + np.random.default_rng(5).random((np_, nn)))
+    Y = np.zeros((np_, nn))
+    if case == "N":
+        for i, o, m in lst:
+            if abs(V[m]) >= 1e-12:
+                Y[:, o] += V[m] * X[:, i]
+    elif case == "P":
+        for i, o, m in lst:
+            if abs(V[m]) >= 1e-12:
+                Y[o, :] += V[m] * X[i, :]
+    else:
+        for ni, no, mn in lists[0]:
+            for pi, po, mp in lists[1]:
+                Y[po, no] += V[mn + 2 * mp] * X[pi, ni]
+    y = O.matvec(prob, X.ravel())
+    assert np.abs(y - Y.ravel()).max() <= 1e-12 * np.abs(Y).max()
+    if case != "NP":
+        assert (Y == 0).any()   # a skipped product left its output empty
