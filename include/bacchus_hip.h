@@ -83,6 +83,42 @@ int bacchus_hip_operator_load(const char *combination_table_file,
 			      const char *matrix_file_base_directory,
 			      int precision, bacchus_hip_operator_t **op);
 
+/* One evaluation-order instruction as the reference's parser hands it over
+ * (evaluation_instruction_t, src/Minerva/evaluation_order/
+ * evaluation_order.h:22-31): basis blocks, index lists and the element
+ * block by their comb.txt ids; a list id of 0 means "none".  type: 1 N,
+ * 2 P, 3 NP.  num_particles: 1, 2 or 3 (the force class of the element
+ * block for the scales), or 0 when unknown (2-body). */
+typedef struct bacchus_hip_file_instruction {
+	int32_t type;
+	int32_t num_particles;
+	int64_t block_in;
+	int64_t block_out;
+	int64_t neutron_list;
+	int64_t proton_list;
+	int64_t matrix_block;
+} bacchus_hip_file_instruction_t;
+
+/* Constructor from already-parsed tables: replaces new_scheduler
+ * (src/Minerva/scheduler/scheduler.c:62-78, scheduler.h:10-14) for a caller
+ * that holds the reference's combination table and evaluation order.  The
+ * basis blocks are given in vector order (comb.txt order) with their ids;
+ * index_list_<id> and <id>_matrix_elements are read from the two
+ * directories.  maximum_loaded_memory (bytes; 0 = none) is checked as the
+ * reference's memory manager checks it: an instruction whose vector
+ * blocks, element block and lists exceed it is an error
+ * (src/Minerva/memory_manager/memory_manager.c:354-365); the operator is
+ * kept resident in device memory, so one whose device tables exceed it is
+ * an error too (there is no out-of-core mode). */
+int bacchus_hip_operator_load_blocks(int64_t num_blocks, const int64_t *block_ids,
+				     const int64_t *block_np, const int64_t *block_nn,
+				     int64_t num_instructions,
+				     const bacchus_hip_file_instruction_t *instructions,
+				     const char *index_lists_base_directory,
+				     const char *matrix_file_base_directory,
+				     size_t maximum_loaded_memory, int precision,
+				     bacchus_hip_operator_t **op);
+
 void bacchus_hip_operator_free(bacchus_hip_operator_t *op);
 
 int64_t bacchus_hip_operator_dimension(const bacchus_hip_operator_t *op);
@@ -119,7 +155,7 @@ int bacchus_hip_operator_stats(const bacchus_hip_operator_t *op,
 /* Drop-in for run_matrix_vector_multiplication
  * (src/Minerva/scheduler/scheduler.c:80-145): reads <in>/vec_<id>, adds
  * H x to the existing <out>/vec_<id> files (the reference loads the output
- * files and accumulates, src/Minerva/vector_block/vector_block.c:274-294)
+ * files and accumulates, src/Minerva/vector_block/vector_block.c:47-67)
  * and writes them back. */
 int bacchus_hip_run_matrix_vector_multiplication(
 	const char *output_vector_base_directory,
@@ -135,10 +171,17 @@ enum bacchus_hip_reorth { BACCHUS_HIP_REORTH_REFERENCE = 0, /* MGS vs
 			  BACCHUS_HIP_REORTH_FULL = 1,     /* CGS2 vs all */
 			  BACCHUS_HIP_REORTH_NONE = 2 };
 enum bacchus_hip_convergence { BACCHUS_HIP_CONVERGE_EIGENVALUES = 0,
+			       /* |a - a'| of the target eigenvector's Krylov
+				  amplitudes between iterations (lanczos.c:
+				  276-287 difference_eigenvectors) */
+			       BACCHUS_HIP_CONVERGE_EIGENVECTORS = 1,
 			       BACCHUS_HIP_CONVERGE_NONE = 2,
 			       BACCHUS_HIP_CONVERGE_TARGETS = 3,
 			       /* block solver: |r_k|^2 / gap_k <= tol |theta_k|
-				  for the lowest num_eigenvalues Ritz pairs */
+				  for the lowest num_eigenvalues Ritz pairs, with
+				  gap_k = min_{i != k} |theta_i - theta_k| - |r_i|
+				  (or |r_k| <= tol |theta_k| where that gap is
+				  not positive) */
 			       BACCHUS_HIP_CONVERGE_RESIDUALS = 4 };
 
 typedef struct bacchus_hip_lanczos_settings {
@@ -154,6 +197,11 @@ typedef struct bacchus_hip_lanczos_settings {
 					     lowest num_eigenvalues are within
 					     eigenvalue_tolerance (relative) */
 	int32_t check_every;      /* iterations between eigenvalue checks */
+	int32_t target_eigenvalue; /* index (ascending) of the eigenvalue the
+				      EIGENVALUES / EIGENVECTORS criteria
+				      watch (lanczos.h:23, bacchus.c:55) */
+	int32_t num_ritz_vectors; /* Ritz vectors written to ritz_vector
+				     (0 = 1)                             */
 	int32_t pad;
 } bacchus_hip_lanczos_settings_t;
 
@@ -161,12 +209,18 @@ typedef struct bacchus_hip_lanczos_result {
 	int64_t iterations;       /* Krylov dimension reached            */
 	double matvec_seconds;    /* device time in the matvec            */
 	double total_seconds;     /* wall time of the solve               */
+	int32_t converged;        /* 1: the convergence criterion stopped
+				     the solve; 0: the iteration (block)
+				     limit did                            */
+	int32_t pad;
 } bacchus_hip_lanczos_result_t;
 
 /* Runs Lanczos; writes the lowest num_eigenvalues Ritz values to
  * eigenvalues (host), alpha/beta (host, max_num_iterations each, may be
- * NULL), and, if ritz_vector is non-NULL (device, D doubles), the Ritz
- * vector of the lowest eigenvalue. */
+ * NULL), and, if ritz_vector is non-NULL (device, num_ritz_vectors x D
+ * doubles), the Ritz vectors of the lowest num_ritz_vectors eigenvalues
+ * (ascending, unit norm, Krylov amplitudes signed so that the first is
+ * non-negative as diagonalization.c:55-58 does). */
 int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 			const bacchus_hip_lanczos_settings_t *settings,
 			double *eigenvalues, double *alpha, double *beta,

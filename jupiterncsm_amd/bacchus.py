@@ -35,7 +35,8 @@ FP64, FP32, BF16 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16}
 START_E0, START_RANDOM, START_GIVEN = 0, 1, 2
 REORTH_REFERENCE, REORTH_FULL, REORTH_NONE = 0, 1, 2
-CONVERGE_EIGENVALUES, CONVERGE_NONE, CONVERGE_TARGETS, CONVERGE_RESIDUALS = 0, 2, 3, 4
+CONVERGE_EIGENVALUES, CONVERGE_EIGENVECTORS, CONVERGE_NONE, CONVERGE_TARGETS, CONVERGE_RESIDUALS = \
+    0, 1, 2, 3, 4
 
 EXPORTED_SYMBOLS = (
     "bacchus_hip_operator_create", "bacchus_hip_operator_load", "bacchus_hip_operator_free",
@@ -50,7 +51,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_batch_lanczos", "bacchus_hip_basis_gram", "bacchus_hip_basis_project",
     "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
     "bacchus_hip_write_evaluation_order", "bacchus_hip_block_lanczos",
-    "bacchus_hip_symmetric_eigen",
+    "bacchus_hip_symmetric_eigen", "bacchus_hip_operator_load_blocks",
 )
 
 
@@ -77,6 +78,7 @@ class _LanczosSettings(ctypes.Structure):
         ("num_eigenvalues", ctypes.c_int32), ("eigenvalue_tolerance", ctypes.c_double),
         ("seed", ctypes.c_uint64), ("start_vector", ctypes.c_void_p),
         ("target_eigenvalues", ctypes.c_void_p), ("check_every", ctypes.c_int32),
+        ("target_eigenvalue", ctypes.c_int32), ("num_ritz_vectors", ctypes.c_int32),
         ("pad", ctypes.c_int32),
     ]
 
@@ -102,7 +104,8 @@ class _BlockLanczosSettings(ctypes.Structure):
 
 class _LanczosResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("matvec_seconds", ctypes.c_double),
-                ("total_seconds", ctypes.c_double)]
+                ("total_seconds", ctypes.c_double), ("converged", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 def library_path() -> str:
@@ -338,6 +341,7 @@ class LanczosResult:
     iterations: int
     matvec_seconds: float
     total_seconds: float
+    converged: bool = False     # the criterion (not the iteration limit) stopped it
 
 
 @dataclass
@@ -347,6 +351,7 @@ class BlockLanczosResult:
     blocks: int                 # block steps (batched matvecs) taken
     matvec_seconds: float
     total_seconds: float
+    converged: bool = False     # the criterion (not max_num_blocks) stopped it
 
 
 def symmetric_eigen(A):
@@ -365,9 +370,13 @@ def symmetric_eigen(A):
 def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
                 start=START_E0, reorthogonalization=REORTH_REFERENCE,
                 convergence=CONVERGE_NONE, tolerance=1e-7, seed=1, start_vector=None,
-                target_eigenvalues=None, check_every=1, ritz_vector=None) -> LanczosResult:
+                target_eigenvalues=None, check_every=1, ritz_vector=None,
+                target_eigenvalue=0, num_ritz_vectors=1) -> LanczosResult:
     """Lanczos on the device.  Defaults reproduce the reference
-    (e_0 start, MGS against u_0..u_{k-3}, lanczos.c:98-291)."""
+    (e_0 start, MGS against u_0..u_{k-3}, lanczos.c:60-253).  ritz_vector:
+    float64 device tensor of num_ritz_vectors x D (or D for one) receiving
+    the Ritz vectors of the lowest eigenvalues; target_eigenvalue: index
+    the EIGENVALUES / EIGENVECTORS criteria watch (lanczos.h:23)."""
     lib = library()
     targets = None
     if target_eigenvalues is not None:
@@ -376,7 +385,10 @@ def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
         int(max_num_iterations), start, reorthogonalization, convergence,
         int(num_eigenvalues), float(tolerance), int(seed),
         ctypes.c_void_p(start_vector.data_ptr()) if start_vector is not None else None,
-        _ptr(targets) if targets is not None else None, int(check_every), 0)
+        _ptr(targets) if targets is not None else None, int(check_every),
+        int(target_eigenvalue), int(num_ritz_vectors), 0)
+    if ritz_vector is not None and ritz_vector.numel() < int(num_ritz_vectors) * matrix.dimension:
+        raise BacchusError(f"ritz_vector must hold {num_ritz_vectors} x {matrix.dimension} doubles")
     kmax = max(1, min(matrix.dimension, int(max_num_iterations)))
     ev = np.zeros(max(1, num_eigenvalues))
     alpha = np.zeros(kmax)
@@ -387,7 +399,7 @@ def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
                                    _ptr(beta), rv, ctypes.byref(res)))
     k = int(res.iterations)
     return LanczosResult(ev[:min(k, num_eigenvalues)], alpha[:k], beta[:k], k,
-                         res.matvec_seconds, res.total_seconds)
+                         res.matvec_seconds, res.total_seconds, bool(res.converged))
 
 
 class Batch:
@@ -522,7 +534,7 @@ class Batch:
         _check(library().bacchus_hip_block_lanczos(
             self.handle, ctypes.byref(st), _ptr(ev), _ptr(rs), rv, ctypes.byref(res)))
         return BlockLanczosResult(ev, rs, int(res.iterations), float(res.matvec_seconds),
-                                  float(res.total_seconds))
+                                  float(res.total_seconds), bool(res.converged))
 
     def close(self):
         if self._h is not None and self._h.value:

@@ -159,6 +159,7 @@ struct bacchus_hip_operator {
 	double mults = 0;
 	int num_cu = 256;
 	// statistics
+	size_t device_bytes = 0;        // device memory the operator holds
 	double algorithmic_bytes = 0;
 	double flops = 0;
 	int64_t element_count = 0;

@@ -71,6 +71,20 @@ __device__ __forceinline__ void fma4(float4 &a, float v, const float4 &x)
 	a.w += v * x.w;
 }
 
+// a += v x for the vectors whose scaled element |s v| is not below 1e-12:
+// N and P instructions skip small elements per vector, on the element
+// scaled by that vector's force-class scale, as the reference does on the
+// (Vulcan-combined) element it reads (matrix_vector_multiplication.c:
+// 50-53, :105-108)
+__device__ __forceinline__ void fma4_kept(float4 &a, float v, const float4 &x, const float4 &s)
+{
+	const float av = fabsf(v);
+	a.x += (av * fabsf(s.x) < 1e-12f) ? 0.f : v * x.x;
+	a.y += (av * fabsf(s.y) < 1e-12f) ? 0.f : v * x.y;
+	a.z += (av * fabsf(s.z) < 1e-12f) ? 0.f : v * x.z;
+	a.w += (av * fabsf(s.w) < 1e-12f) ? 0.f : v * x.w;
+}
+
 // the LDS tile holds FP64 sums (ds_add_f64)
 __device__ __forceinline__ void add4_lds(double *t, const float4 &s, const float4 &v)
 {
@@ -102,7 +116,8 @@ template <> struct Load<uint16_t> {
 //   P (:79-132): slots over columns, same broadcast; for narrow rows,
 //     slots of column groups each walking its own entries.
 //   N (:20-77) runs in the transposed space as P there.
-// N and P skip |V| < 1e-12 as the reference does (:52, :107).
+// N and P skip elements with |s V| < 1e-12 per vector (s its force-class
+// scale), as the reference does on the element it reads (:50-53, :105-108).
 template <typename VT>
 __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 			   double *tile)
@@ -267,8 +282,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						a = make_float4(0.f, 0.f, 0.f, 0.f);
 						cur = rw[u];
 					}
-					if (!(fabsf(e[u]) < 1e-12f))
-						fma4(a, sign_of(e[u], en[u].y), xv[u]);
+					fma4_kept(a, sign_of(e[u], en[u].y), xv[u], sc);
 				}
 			}
 			if (cur >= 0 && valid)
@@ -296,8 +310,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
 						const float e = Load<VT>::at(V, (size_t)I.v_off + (py & kMask));
-						if (!(fabsf(e) < 1e-12f))
-							fma4(a0, sign_of(e, py), ld4(xb + ((size_t)nn_in * px + nl) * B));
+						fma4_kept(a0, sign_of(e, py), ld4(xb + ((size_t)nn_in * px + nl) * B), sc);
 					}
 				}
 				if (cur >= 0 && valid)
@@ -1257,6 +1270,7 @@ extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
 	const bool converge = set->eigenvalue_tolerance > 0;
 	std::vector<std::vector<double>> prev(nops);
 	int64_t k = 0;
+	bool converged = false;
 	std::vector<std::vector<double>> ha(nops), hb(nops);
 	auto read_ab = [&](int64_t kk) -> int {
 		for (int i = 0; i < nops; i++) {
@@ -1302,8 +1316,10 @@ extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
 						all = false;
 			prev[i] = cur;
 		}
-		if (all)
+		if (all) {
+			converged = true;
 			break;
+		}
 	}
 	if (int rc = read_ab(k))
 		return rc;
@@ -1349,6 +1365,8 @@ extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
 	BACCHUS_HIP_CHECK(hipStreamSynchronize(L.s));
 	if (result) {
 		result->iterations = k;
+		result->converged = converged ? 1 : 0;
+		result->pad = 0;
 		result->matvec_seconds = 0;
 		result->total_seconds =
 			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -492,6 +492,7 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 	std::vector<double> T((size_t)(K + b) * (K + b), 0.0);
 	std::vector<double> theta, S, res(nev, 0.0);
 	int done = 0;
+	bool converged = false;
 	for (int j = 0; j < J; j++) {
 		// W = H Q_j
 		BACCHUS_HIP_CHECK(hipMemcpy(L.X, L.blk(j), (size_t)D * b * sizeof(float),
@@ -531,8 +532,10 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 			for (int q = 0; q < m; q++)
 				Tm[(size_t)p * m + q] = T[(size_t)p * ld + q];
 		jacobi_eigen(m, Tm, theta, S);
-		// residual of Ritz pair k: |R S[j b .. (j+1) b, k]|
-		for (int k = 0; k < nev && k < m; k++) {
+		// residual of Ritz pair k: |R S[j b .. (j+1) b, k]| (every pair:
+		// the gap of the bound below subtracts the others' residuals)
+		res.assign(m, 0.0);
+		for (int k = 0; k < m; k++) {
 			double s2 = 0;
 			for (int p = 0; p < b; p++) {
 				double s = 0;
@@ -549,16 +552,23 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 				const double t = set->target_eigenvalues[k];
 				ok = ok && std::fabs(theta[k] - t) <= set->tolerance * std::fabs(t);
 			} else if (conv == BACCHUS_HIP_CONVERGE_RESIDUALS) {
+				// |theta_k - lambda| <= |r_k|^2 / gap_k, gap_k a lower bound
+				// of the distance from theta_k to the rest of the spectrum:
+				// every other Ritz value theta_i has an eigenvalue within
+				// |r_i| of it, so gap_k = min_i (|theta_i - theta_k| - |r_i|).
+				// Where that is not positive, the plain residual bound
+				// |theta_k - lambda| <= |r_k| decides.
 				double gap = INFINITY;
 				for (int i = 0; i < m; i++)
 					if (i != k)
-						gap = std::min(gap, std::fabs(theta[i] - theta[k]));
+						gap = std::min(gap, std::fabs(theta[i] - theta[k]) - res[i]);
 				const double bound = std::isfinite(gap) && gap > 0 ? res[k] * res[k] / gap : res[k];
 				ok = ok && bound <= set->tolerance * std::fabs(theta[k]);
 			} else {
 				ok = false;
 			}
 		}
+		converged = ok;
 		if (ok || j + 1 == J)
 			break;
 		if (int rc = L.second_round(j + 1, L.blk(j + 1), coef, R, seed_j))
@@ -583,6 +593,8 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
 	}
 	if (result) {
 		result->iterations = done;
+		result->converged = converged ? 1 : 0;
+		result->pad = 0;
 		result->matvec_seconds = L.matvec_ms * 1e-3;
 		result->total_seconds =
 			std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

@@ -250,6 +250,148 @@ std::vector<CalcBlock> mars_order(const CombTable &t, bool two_nucleon_only)
 	return need;
 }
 
+// One instruction with the reference's file ids: basis blocks as dense
+// indices, index lists and the element block by id (0 = no list).
+struct IdInstruction {
+	int32_t type;
+	int32_t block_in, block_out;
+	int64_t neutron_list, proton_list, matrix_block;
+};
+
+// Reads the index lists (index_list.c:69-97) and element blocks
+// (matrix_block.c:22-62) the instructions name, checks the memory budget the
+// way the reference's memory manager does (a single instruction needing more
+// than maximum_loaded_memory is an error, memory_manager.c:354-365) and
+// builds the device operator.  maximum_loaded_memory = 0 means no budget.
+int create_from_ids(const std::vector<int64_t> &block_ids, const std::vector<int64_t> &block_np,
+		    const std::vector<int64_t> &block_nn, const std::vector<IdInstruction> &ins,
+		    const std::map<int64_t, int> &matrix_rank, const char *index_dir,
+		    const char *matrix_dir, size_t maximum_loaded_memory, int precision,
+		    bacchus_hip_operator_t **op)
+{
+	std::vector<bacchus_hip_instruction_t> instr;
+	std::map<int64_t, int32_t> list_index, v_index;
+	std::vector<int64_t> list_ids, v_ids;
+	auto dense = [](std::map<int64_t, int32_t> &m, std::vector<int64_t> &ids, int64_t id) {
+		auto it = m.find(id);
+		if (it != m.end())
+			return it->second;
+		const int32_t k = (int32_t)ids.size();
+		ids.push_back(id);
+		m[id] = k;
+		return k;
+	};
+	const int64_t nb = (int64_t)block_ids.size();
+	for (const IdInstruction &w : ins) {
+		if (w.block_in < 0 || w.block_in >= nb || w.block_out < 0 || w.block_out >= nb)
+			return BACCHUS_FAIL("instruction names an unknown basis block");
+		if (w.type < 1 || w.type > 3)
+			return BACCHUS_FAIL("instruction has unknown type %d", w.type);
+		bacchus_hip_instruction_t I{};
+		I.type = w.type;
+		I.block_in = w.block_in;
+		I.block_out = w.block_out;
+		I.neutron_list = w.type != 2 ? dense(list_index, list_ids, w.neutron_list) : -1;
+		I.proton_list = w.type != 1 ? dense(list_index, list_ids, w.proton_list) : -1;
+		I.matrix_block = dense(v_index, v_ids, w.matrix_block);
+		instr.push_back(I);
+	}
+	// index lists
+	std::vector<int64_t> list_offsets{0};
+	std::vector<int32_t> list_entries;
+	std::vector<char> buf;
+	for (int64_t id : list_ids) {
+		char name[64];
+		snprintf(name, sizeof name, "/index_list_%lld", (long long)id);
+		if (read_file(std::string(index_dir) + name, buf))
+			return 1;
+		if (buf.size() % 12 != 0)
+			return BACCHUS_FAIL("index list %lld is not a whole number of "
+					    "triples", (long long)id);
+		const size_t n = buf.size() / 4;
+		const size_t old = list_entries.size();
+		list_entries.resize(old + n);
+		if (n)
+			memcpy(&list_entries[old], buf.data(), buf.size());
+		list_offsets.push_back((int64_t)list_entries.size() / 3);
+	}
+	// matrix elements: size_t Dn, size_t Dp, double[n] with n = Dn*Dp, or Dp
+	// if Dn = 0, or Dn if Dp = 0
+	std::vector<int64_t> v_offsets{0}, v_dn;
+	std::vector<int32_t> v_force;
+	std::vector<double> elements;
+	for (int64_t id : v_ids) {
+		char name[64];
+		snprintf(name, sizeof name, "/%lld_matrix_elements", (long long)id);
+		if (read_file(std::string(matrix_dir) + name, buf))
+			return 1;
+		if (buf.size() < 16)
+			return BACCHUS_FAIL("Could not read neutron_matrix_dimension from "
+					    "matrix file %lld", (long long)id);
+		uint64_t dn, dp;
+		memcpy(&dn, buf.data(), 8);
+		memcpy(&dp, buf.data() + 8, 8);
+		const uint64_t n = dn == 0 ? dp : (dp == 0 ? dn : dn * dp);
+		if (buf.size() < 16 + 8 * n)
+			return BACCHUS_FAIL("Could not read matrix elements form matrix "
+					    "file %lld", (long long)id);
+		const size_t old = elements.size();
+		elements.resize(old + n);
+		if (n)
+			memcpy(&elements[old], buf.data() + 16, 8 * n);
+		v_offsets.push_back((int64_t)elements.size());
+		v_dn.push_back((int64_t)dn);
+		auto r = matrix_rank.find(id);
+		v_force.push_back(r == matrix_rank.end() ? 2 : std::min(3, std::max(1, r->second)));
+	}
+	// the budget, per instruction: its vector blocks, element block and
+	// index lists (the arrays get_size_of_unloaded_arrays adds up,
+	// memory_manager.c:328-352)
+	if (maximum_loaded_memory) {
+		for (size_t k = 0; k < instr.size(); k++) {
+			const bacchus_hip_instruction_t &I = instr[k];
+			size_t need = 8 * (size_t)(block_np[I.block_in] * block_nn[I.block_in]);
+			if (I.block_out != I.block_in)
+				need += 8 * (size_t)(block_np[I.block_out] * block_nn[I.block_out]);
+			need += 8 * (size_t)(v_offsets[I.matrix_block + 1] - v_offsets[I.matrix_block]);
+			for (int32_t l : {I.neutron_list, I.proton_list})
+				if (l >= 0)
+					need += 12 * (size_t)(list_offsets[l + 1] - list_offsets[l]);
+			if (need > maximum_loaded_memory)
+				return BACCHUS_FAIL("Block %zu needs %zu B to be loaded,But maximaly "
+						    "allowed loaded memory is %zu.", k, need,
+						    maximum_loaded_memory);
+		}
+	}
+	bacchus_hip_problem_t p{};
+	p.num_blocks = nb;
+	p.block_np = block_np.data();
+	p.block_nn = block_nn.data();
+	p.num_lists = (int64_t)list_ids.size();
+	p.list_offsets = list_offsets.data();
+	p.list_entries = list_entries.data();
+	p.num_matrix_blocks = (int64_t)v_ids.size();
+	p.matrix_offsets = v_offsets.data();
+	p.matrix_dn = v_dn.data();
+	p.matrix_force = v_force.data();
+	p.matrix_elements = elements.data();
+	p.num_instructions = (int64_t)instr.size();
+	p.instructions = instr.data();
+	const int rc = bacchus_hip_operator_create(&p, precision, op);
+	if (rc)
+		return rc;
+	(*op)->block_ids = block_ids;
+	if (maximum_loaded_memory && (*op)->device_bytes > maximum_loaded_memory) {
+		const size_t have = (*op)->device_bytes;
+		bacchus_hip_operator_free(*op);
+		*op = nullptr;
+		return BACCHUS_FAIL("the operator needs %zu B of device memory, more than the "
+				    "maximum_loaded_memory of %zu B (it is kept resident; there is "
+				    "no out-of-core mode)", have, maximum_loaded_memory);
+	}
+	return 0;
+}
+
 }  // namespace
 }  // namespace bacchus
 
@@ -278,18 +420,7 @@ int bacchus_hip_operator_load(const char *combination_table_file,
 	if (!read_lines(evaluation_order_file, rows))
 		return BACCHUS_FAIL("Could not open evaluation order file %s. %s",
 				    evaluation_order_file, strerror(errno));
-	std::vector<bacchus_hip_instruction_t> instr;
-	std::map<int64_t, int32_t> list_index, v_index;
-	std::vector<int64_t> list_ids, v_ids;
-	auto dense = [](std::map<int64_t, int32_t> &m, std::vector<int64_t> &ids, int64_t id) {
-		auto it = m.find(id);
-		if (it != m.end())
-			return it->second;
-		const int32_t k = (int32_t)ids.size();
-		ids.push_back(id);
-		m[id] = k;
-		return k;
-	};
+	std::vector<IdInstruction> instr;
 	for (const std::string &row : rows) {
 		const size_t at = row.find("BLOCK:");
 		if (at == std::string::npos || row.find("UNLOAD_") != std::string::npos)
@@ -298,7 +429,7 @@ int bacchus_hip_operator_load(const char *combination_table_file,
 		if (w.size() != 5 && w.size() != 6)
 			return BACCHUS_FAIL("evaluation order row \"%s\" is not formatted "
 					    "correctly", row.c_str());
-		bacchus_hip_instruction_t I{};
+		IdInstruction I{};
 		const int64_t bin = atoll(w[1].c_str()), bout = atoll(w[2].c_str());
 		if (!comb.block_index.count(bin) || !comb.block_index.count(bout))
 			return BACCHUS_FAIL("evaluation order row \"%s\" names an unknown "
@@ -317,9 +448,9 @@ int bacchus_hip_operator_load(const char *combination_table_file,
 			if (type_of(l1) != 'n')
 				std::swap(l1, l2);
 			I.type = 3;
-			I.neutron_list = dense(list_index, list_ids, l1);
-			I.proton_list = dense(list_index, list_ids, l2);
-			I.matrix_block = dense(v_index, v_ids, atoll(w[5].c_str()));
+			I.neutron_list = l1;
+			I.proton_list = l2;
+			I.matrix_block = atoll(w[5].c_str());
 		} else {
 			const int64_t l = atoll(w[3].c_str());
 			const char c = type_of(l);
@@ -327,79 +458,56 @@ int bacchus_hip_operator_load(const char *combination_table_file,
 				return BACCHUS_FAIL("row \"%s\" names an unknown index list",
 						    row.c_str());
 			I.type = c == 'n' ? 1 : 2;
-			I.neutron_list = c == 'n' ? dense(list_index, list_ids, l) : -1;
-			I.proton_list = c == 'n' ? -1 : dense(list_index, list_ids, l);
-			I.matrix_block = dense(v_index, v_ids, atoll(w[4].c_str()));
+			I.neutron_list = c == 'n' ? l : 0;
+			I.proton_list = c == 'n' ? 0 : l;
+			I.matrix_block = atoll(w[4].c_str());
 		}
 		instr.push_back(I);
 	}
-	// index lists (index_list.c:69-97)
-	std::vector<int64_t> list_offsets{0};
-	std::vector<int32_t> list_entries;
-	std::vector<char> buf;
-	for (int64_t id : list_ids) {
-		char name[64];
-		snprintf(name, sizeof name, "/index_list_%lld", (long long)id);
-		if (read_file(std::string(index_lists_base_directory) + name, buf))
-			return 1;
-		if (buf.size() % 12 != 0)
-			return BACCHUS_FAIL("index list %lld is not a whole number of "
-					    "triples", (long long)id);
-		const size_t n = buf.size() / 4;
-		const size_t old = list_entries.size();
-		list_entries.resize(old + n);
-		if (n)
-			memcpy(&list_entries[old], buf.data(), buf.size());
-		list_offsets.push_back((int64_t)list_entries.size() / 3);
+	return create_from_ids(comb.block_ids, comb.block_np, comb.block_nn, instr,
+			       comb.matrix_rank, index_lists_base_directory,
+			       matrix_file_base_directory, 0, precision, op);
+}
+
+int bacchus_hip_operator_load_blocks(int64_t num_blocks, const int64_t *block_ids,
+				     const int64_t *block_np, const int64_t *block_nn,
+				     int64_t num_instructions,
+				     const bacchus_hip_file_instruction_t *instructions,
+				     const char *index_lists_base_directory,
+				     const char *matrix_file_base_directory,
+				     size_t maximum_loaded_memory, int precision,
+				     bacchus_hip_operator_t **op)
+{
+	if (!op || !block_ids || !block_np || !block_nn || (num_instructions && !instructions) ||
+	    !index_lists_base_directory || !matrix_file_base_directory || num_blocks <= 0 ||
+	    num_instructions < 0)
+		return BACCHUS_FAIL("NULL or invalid argument");
+	*op = nullptr;
+	std::vector<int64_t> ids(block_ids, block_ids + num_blocks);
+	std::vector<int64_t> np(block_np, block_np + num_blocks), nn(block_nn, block_nn + num_blocks);
+	std::map<int64_t, int64_t> index;
+	for (int64_t b = 0; b < num_blocks; b++)
+		index[ids[b]] = b;
+	std::map<int64_t, int> rank;
+	std::vector<IdInstruction> instr;
+	for (int64_t k = 0; k < num_instructions; k++) {
+		const bacchus_hip_file_instruction_t &f = instructions[k];
+		if (!index.count(f.block_in) || !index.count(f.block_out))
+			return BACCHUS_FAIL("instruction %lld names an unknown basis block",
+					    (long long)k);
+		IdInstruction I{};
+		I.type = f.type;
+		I.block_in = (int32_t)index[f.block_in];
+		I.block_out = (int32_t)index[f.block_out];
+		I.neutron_list = f.neutron_list;
+		I.proton_list = f.proton_list;
+		I.matrix_block = f.matrix_block;
+		if (f.num_particles >= 1)
+			rank[f.matrix_block] = f.num_particles;
+		instr.push_back(I);
 	}
-	// matrix elements (matrix_block.c:22-62): size_t Dn, size_t Dp,
-	// double[n] with n = Dn*Dp, or Dp if Dn = 0, or Dn if Dp = 0
-	std::vector<int64_t> v_offsets{0}, v_dn;
-	std::vector<int32_t> v_force;
-	std::vector<double> elements;
-	for (int64_t id : v_ids) {
-		char name[64];
-		snprintf(name, sizeof name, "/%lld_matrix_elements", (long long)id);
-		if (read_file(std::string(matrix_file_base_directory) + name, buf))
-			return 1;
-		if (buf.size() < 16)
-			return BACCHUS_FAIL("Could not read neutron_matrix_dimension from "
-					    "matrix file %lld", (long long)id);
-		uint64_t dn, dp;
-		memcpy(&dn, buf.data(), 8);
-		memcpy(&dp, buf.data() + 8, 8);
-		const uint64_t n = dn == 0 ? dp : (dp == 0 ? dn : dn * dp);
-		if (buf.size() < 16 + 8 * n)
-			return BACCHUS_FAIL("Could not read matrix elements form matrix "
-					    "file %lld", (long long)id);
-		const size_t old = elements.size();
-		elements.resize(old + n);
-		if (n)
-			memcpy(&elements[old], buf.data() + 16, 8 * n);
-		v_offsets.push_back((int64_t)elements.size());
-		v_dn.push_back((int64_t)dn);
-		auto r = comb.matrix_rank.find(id);
-		v_force.push_back(r == comb.matrix_rank.end() ? 2 : std::min(3, std::max(1, r->second)));
-	}
-	bacchus_hip_problem_t p{};
-	p.num_blocks = (int64_t)comb.block_ids.size();
-	p.block_np = comb.block_np.data();
-	p.block_nn = comb.block_nn.data();
-	p.num_lists = (int64_t)list_ids.size();
-	p.list_offsets = list_offsets.data();
-	p.list_entries = list_entries.data();
-	p.num_matrix_blocks = (int64_t)v_ids.size();
-	p.matrix_offsets = v_offsets.data();
-	p.matrix_dn = v_dn.data();
-	p.matrix_force = v_force.data();
-	p.matrix_elements = elements.data();
-	p.num_instructions = (int64_t)instr.size();
-	p.instructions = instr.data();
-	const int rc = bacchus_hip_operator_create(&p, precision, op);
-	if (rc)
-		return rc;
-	(*op)->block_ids = comb.block_ids;
-	return 0;
+	return create_from_ids(ids, np, nn, instr, rank, index_lists_base_directory,
+			       matrix_file_base_directory, maximum_loaded_memory, precision, op);
 }
 
 int bacchus_hip_run_matrix_vector_multiplication(

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void reduce_partials(
 }
 
 // w -= alpha_k u_k (+ beta_{k-1} u_{k-1}) with alpha, beta read on device
-// (lanczos.c:151-173)
+// (lanczos.c:113-135)
 __global__ void three_term(double *__restrict__ w, const double *__restrict__ uk,
 			   const double *__restrict__ ukm1, const double *alpha_k,
 			   const double *beta_km1, int64_t n)
@@ -119,7 +119,7 @@ __global__ void subtract_one(double *__restrict__ w, const double *__restrict__ 
 		w[i] -= (*c) * u[i];
 }
 
-// beta = sqrt(ss); w *= 1/beta (lanczos.c:175-179)
+// beta = sqrt(ss); w *= 1/beta (lanczos.c:137-141)
 __global__ void finish_beta(const double *ss, double *beta_k)
 {
 	*beta_k = sqrt(*ss);
@@ -353,9 +353,21 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 	const auto t0 = std::chrono::steady_clock::now();
 	const int64_t D = op->dimension;
 	const int64_t K = std::min<int64_t>(D, st->max_num_iterations);
-	const int m = std::max(1, st->num_eigenvalues);
+	const int nritz = std::max(1, st->num_ritz_vectors);
+	const int target = st->target_eigenvalue;
+	// eigenvalues computed per check: the reported ones, the Ritz vectors'
+	// and the target's
+	const int m = std::max({1, st->num_eigenvalues, nritz, target + 1});
 	if (K < 1)
 		return BACCHUS_FAIL("max_num_iterations must be positive");
+	if (target < 0)
+		return BACCHUS_FAIL("target_eigenvalue %d is negative", target);
+	if (st->convergence != BACCHUS_HIP_CONVERGE_EIGENVALUES &&
+	    st->convergence != BACCHUS_HIP_CONVERGE_EIGENVECTORS &&
+	    st->convergence != BACCHUS_HIP_CONVERGE_NONE &&
+	    st->convergence != BACCHUS_HIP_CONVERGE_TARGETS)
+		return BACCHUS_FAIL("convergence %d not supported by the Lanczos solver",
+				    st->convergence);
 	if (st->start == BACCHUS_HIP_START_GIVEN && !st->start_vector)
 		return BACCHUS_FAIL("start vector missing");
 	if (st->convergence == BACCHUS_HIP_CONVERGE_TARGETS && !st->target_eigenvalues)
@@ -396,8 +408,12 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 	double matvec_ms = 0;
 	int64_t it = 0;
 	double previous = 0;
+	// EIGENVECTORS: the target eigenvector's amplitudes of the previous
+	// check, zero beyond its length (lanczos.c:183-184, 238-240)
+	std::vector<double> previous_amplitudes(K + 1, 0.0);
 	std::vector<double> h_eig(m), h_alpha(K), h_beta(K);
 	const int check_every = std::max(1, st->check_every);
+	bool converged = false;
 	for (it = 0; it < K && !rc; it++) {
 		double *u = U + it * D, *w = U + (it + 1) * D;
 		(void)hipEventRecord(ev[2 * it], L.s);
@@ -435,16 +451,39 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 			const int cnt = std::min<int64_t>(m, it + 1);
 			hipLaunchKernelGGL(tridiagonal_lowest, dim3(1), dim3(kThreads), 0, L.s,
 					   d_alpha, d_beta, (int)(it + 1), cnt, d_eig);
+			const bool amplitudes = st->convergence == BACCHUS_HIP_CONVERGE_EIGENVECTORS;
 			if (hipMemcpyAsync(h_eig.data(), d_eig, cnt * sizeof(double),
 					   hipMemcpyDeviceToHost, L.s) != hipSuccess ||
+			    (amplitudes &&
+			     (hipMemcpyAsync(h_alpha.data(), d_alpha, (it + 1) * sizeof(double),
+					     hipMemcpyDeviceToHost, L.s) != hipSuccess ||
+			      hipMemcpyAsync(h_beta.data(), d_beta, (it + 1) * sizeof(double),
+					     hipMemcpyDeviceToHost, L.s) != hipSuccess)) ||
 			    hipStreamSynchronize(L.s) != hipSuccess) {
 				rc = BACCHUS_FAIL("eigenvalue readback failed");
 				break;
 			}
 			bool done = false;
+			// the reference watches eigenvalue index target_eigenvalue of
+			// T; while T is smaller, the highest it has (get_eigenvalue
+			// asserts the index, eigensystem.c:60-66)
+			const int tk = std::min<int>(target, cnt - 1);
 			if (st->convergence == BACCHUS_HIP_CONVERGE_EIGENVALUES) {
-				done = std::fabs(h_eig[0] - previous) < st->eigenvalue_tolerance;
-				previous = h_eig[0];
+				done = std::fabs(h_eig[tk] - previous) < st->eigenvalue_tolerance;
+				previous = h_eig[tk];
+			} else if (amplitudes) {
+				// difference_eigenvectors (lanczos.c:276-287): the first
+				// n - 1 amplitudes against the previous ones, the new last
+				// one against zero
+				const int64_t n = it + 1;
+				const std::vector<double> a =
+					tridiagonal_vector(h_alpha, h_beta, n, h_eig[tk]);
+				double d2 = 0;
+				for (int64_t i = 0; i + 1 < n; i++)
+					d2 += (a[i] - previous_amplitudes[i]) * (a[i] - previous_amplitudes[i]);
+				d2 += a[n - 1] * a[n - 1];
+				done = std::sqrt(d2) < st->eigenvalue_tolerance;
+				std::copy(a.begin(), a.end(), previous_amplitudes.begin());
 			} else if (cnt == m) {
 				done = true;
 				for (int k = 0; k < m; k++)
@@ -453,6 +492,7 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 						done = false;
 			}
 			if (done) {
+				converged = true;
 				it++;
 				break;
 			}
@@ -471,20 +511,30 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 				   hipMemcpyDeviceToHost, L.s) != hipSuccess ||
 		    hipStreamSynchronize(L.s) != hipSuccess)
 			rc = BACCHUS_FAIL("result readback failed");
-		for (int k = 0; k < cnt && !rc; k++)
+		// the caller's array holds num_eigenvalues values (m may be larger
+		// for the Ritz vectors or the target)
+		for (int k = 0; k < cnt && k < std::max(1, st->num_eigenvalues) && !rc; k++)
 			eigenvalues[k] = h_eig[k];
 		if (!rc && alpha_out)
 			memcpy(alpha_out, h_alpha.data(), dim * sizeof(double));
 		if (!rc && beta_out)
 			memcpy(beta_out, h_beta.data(), dim * sizeof(double));
-		if (!rc && ritz_vector) {
-			std::vector<double> z = tridiagonal_vector(h_alpha, h_beta, dim, h_eig[0]);
+		// Ritz vector r = sum_j z_j u_j, z the eigenvector of T for the
+		// r-th lowest eigenvalue (the amplitudes get_eigenvector combines,
+		// eigensystem.c:85-95)
+		for (int r = 0; r < nritz && !rc && ritz_vector; r++) {
+			if (r >= cnt) {
+				rc = BACCHUS_FAIL("%d Ritz vectors asked for, the Krylov space has "
+						  "dimension %lld", nritz, (long long)dim);
+				break;
+			}
+			std::vector<double> z = tridiagonal_vector(h_alpha, h_beta, dim, h_eig[r]);
 			if (hipMemcpy(d_coef, z.data(), dim * sizeof(double), hipMemcpyHostToDevice) !=
 			    hipSuccess)
 				rc = BACCHUS_FAIL("ritz coefficients upload failed");
 			else {
 				hipLaunchKernelGGL(combine_vectors, dim3(gb), dim3(kThreads), 0, L.s,
-						   ritz_vector, U, D, (int)dim, d_coef, D);
+						   ritz_vector + (size_t)r * D, U, D, (int)dim, d_coef, D);
 				if (hipStreamSynchronize(L.s) != hipSuccess)
 					rc = BACCHUS_FAIL("ritz vector failed");
 			}
@@ -502,6 +552,8 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 	(void)hipFree(L.partial);
 	if (result) {
 		result->iterations = dim;
+		result->converged = converged ? 1 : 0;
+		result->pad = 0;
 		result->matvec_seconds = matvec_ms * 1e-3;
 		result->total_seconds =
 			std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -21,7 +21,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <type_traits>
@@ -53,15 +56,26 @@ constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
 
 // fn(k) for every k in [0, n) on up to 16 host threads, indices taken from
 // a shared counter; the calling thread works too (and does everything if no
-// thread can be started).  fn must not throw.
+// thread can be started).  The first exception a call of fn throws (e.g.
+// std::bad_alloc while a table grows) stops the remaining work and is
+// rethrown in the calling thread after every worker has joined.
 template <typename F> void parallel_for(size_t n, F &&fn)
 {
 	const unsigned hw = std::thread::hardware_concurrency();
 	const size_t nthreads = std::min<size_t>(n, std::max(1u, std::min(16u, hw ? hw : 1u)));
 	std::atomic<size_t> next{0};
+	std::exception_ptr failure;
+	std::mutex failure_lock;
 	auto worker = [&]() {
-		for (size_t q; (q = next.fetch_add(1)) < n;)
-			fn(q);
+		try {
+			for (size_t q; (q = next.fetch_add(1)) < n;)
+				fn(q);
+		} catch (...) {
+			std::lock_guard<std::mutex> g(failure_lock);
+			if (!failure)
+				failure = std::current_exception();
+			next.store(n);
+		}
 	};
 	std::vector<std::thread> pool;
 	for (size_t t = 1; t < nthreads; t++) {
@@ -74,6 +88,8 @@ template <typename F> void parallel_for(size_t n, F &&fn)
 	worker();
 	for (auto &t : pool)
 		t.join();
+	if (failure)
+		std::rethrow_exception(failure);
 }
 
 // allocator whose resize() leaves new elements uninitialized: every range
@@ -148,7 +164,9 @@ template <> struct Elem<uint16_t> {
 //     each walking its own entries (narrow rows).
 //   N  (:20-77) runs in the transposed space, where it has the form of P
 //     with the neutron list over the rows and the protons as columns.
-// N and P skip |V| < 1e-12 as the reference does (:52, :107).
+// N and P skip elements with |s V| < 1e-12, s the force-class scale, as the
+// reference does on the (Vulcan-combined) element it reads (:50-53,
+// :105-108).
 template <typename VT>
 __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 	const TileWork *__restrict__ work, int64_t nwork, const Task *__restrict__ tasks,
@@ -422,7 +440,9 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						a = 0;
 						cur = rw[u];
 					}
-					if (!(fabs(e[u]) < acc_t(1e-12)))
+					// the reference skips on the scaled element (a
+					// Vulcan-combined file holds s V)
+					if (!(fabs(sc * e[u]) < acc_t(1e-12)))
 						a += apply_sign(e[u], en[u].y) * xv[u];
 				}
 			}
@@ -457,7 +477,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
 						const acc_t e = Elem<VT>::load(V, I.v_off + (py & kIndexMask));
 						const acc_t xv = acc_t((xblk + (size_t)(nn_in * px))[nl]);
-						if (!(fabs(e) < acc_t(1e-12))) {
+						if (!(fabs(sc * e) < acc_t(1e-12))) {
 							if (t & 1)
 								a1 += apply_sign(e, py) * xv;
 							else
@@ -789,8 +809,26 @@ int bacchus_hip_operator_stats(const bacchus_hip_operator_t *op,
 	return 0;
 }
 
+static int operator_create(const bacchus_hip_problem_t *p, int precision,
+			   bacchus_hip_operator_t **out);
+
+// the tables of a large operator take gigabytes of host memory: a failed
+// allocation while building them comes back as an error code, not as an
+// exception through the C ABI
 int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 				int precision, bacchus_hip_operator_t **out)
+{
+	try {
+		return operator_create(p, precision, out);
+	} catch (const std::bad_alloc &) {
+		return BACCHUS_FAIL("out of host memory while building the operator tables");
+	} catch (const std::exception &e) {
+		return BACCHUS_FAIL("operator tables: %s", e.what());
+	}
+}
+
+static int operator_create(const bacchus_hip_problem_t *p, int precision,
+			   bacchus_hip_operator_t **out)
 {
 	if (!p || !out)
 		return BACCHUS_FAIL("NULL argument");
@@ -802,6 +840,10 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	const int64_t nb = p->num_blocks, nl = p->num_lists,
 		      nv = p->num_matrix_blocks, ni = p->num_instructions;
 	auto *op = new bacchus_hip_operator;
+	// frees the operator (host tables and device memory) on every early
+	// return and exception; released on success
+	std::unique_ptr<bacchus_hip_operator, void (*)(bacchus_hip_operator *)> guard(
+		op, bacchus_hip_operator_free);
 	op->precision = precision;
 	op->block_np.assign(p->block_np, p->block_np + nb);
 	op->block_nn.assign(p->block_nn, p->block_nn + nb);
@@ -812,7 +854,6 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 	for (int64_t b = 0; b < nb; b++) {
 		if (op->block_np[b] <= 0 || op->block_nn[b] <= 0 ||
 		    op->block_np[b] * op->block_nn[b] > INT32_MAX) {
-			delete op;
 			return BACCHUS_FAIL("basis block %lld has invalid size %lldx%lld",
 					    (long long)b, (long long)op->block_np[b],
 					    (long long)op->block_nn[b]);
@@ -820,10 +861,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		op->block_off[b + 1] = op->block_off[b] + op->block_np[b] * op->block_nn[b];
 	}
 	op->dimension = op->block_off[nb];
-	auto fail = [&](int code) {
-		bacchus_hip_operator_free(op);
-		return code;
-	};
+	auto fail = [](int code) { return code; };
 
 	// --- host validation: every index a kernel will touch is in bounds ---
 	std::vector<ListUse> use(nl);
@@ -1431,6 +1469,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		if (bytes == 0)
 			bytes = 16;
 		BACCHUS_HIP_CHECK(hipMalloc(dst, bytes));
+		op->device_bytes += bytes;
 		if (src)
 			BACCHUS_HIP_CHECK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
 		return 0;
@@ -1484,7 +1523,7 @@ int bacchus_hip_operator_create(const bacchus_hip_problem_t *p,
 		    (rc = upload((void **)&op->d_blk_nn, bnn.data(), nb * sizeof(int32_t))))
 			return fail(rc);
 	}
-	*out = op;
+	*out = guard.release();
 	return 0;
 }
 

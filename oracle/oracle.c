@@ -157,8 +157,8 @@ int oracle_matvec(int64_t num_blocks, const int64_t *block_np,
 	const int64_t D = off[num_blocks];
 	if (num_threads < 1)
 		num_threads = 1;
-	/* one output replica per thread (vector_block.c:274-294), reduced
-	 * into y at the end (vector_block.c:372-382) */
+	/* one output replica per thread (vector_block.c:47-67), reduced
+	 * into y at the end (vector_block.c:146-155) */
 	double *rep = (double *)calloc((size_t)num_threads * (size_t)D, sizeof(double));
 	if (!rep) {
 		free(off);
@@ -272,7 +272,7 @@ static double dot(const double *a, const double *b, int64_t n)
 	return s;
 }
 
-/* lanczos.c:98-291; vector ops vector.c:240-476, math_tools.c:13-57 */
+/* lanczos.c:60-253; vector ops vector.c:240-476, math_tools.c:13-57 */
 int64_t oracle_lanczos(int64_t D, oracle_matvec_fn matvec, void *ctx,
 		       int64_t max_num_iterations, int start,
 		       const double *start_vector, int reorth, int convergence,
@@ -289,7 +289,7 @@ int64_t oracle_lanczos(int64_t D, oracle_matvec_fn matvec, void *ctx,
 		free(ev);
 		return -1;
 	}
-	/* lanczos.c:98-110: u_0 = e_0 */
+	/* lanczos.c:60-72: u_0 = e_0 */
 	if (start == ORACLE_START_GIVEN && start_vector)
 		memcpy(U, start_vector, (size_t)D * sizeof(double));
 	else
@@ -298,13 +298,13 @@ int64_t oracle_lanczos(int64_t D, oracle_matvec_fn matvec, void *ctx,
 	int64_t it;
 	for (it = 0; it < K; it++) {
 		double *u = U + (size_t)it * D, *w = U + (size_t)(it + 1) * D;
-		matvec(ctx, u, w); /* lanczos.c:136-138, w starts at zero */
-		const double a = dot(u, w, D); /* :146-149 */
+		matvec(ctx, u, w); /* lanczos.c:98-100, w starts at zero */
+		const double a = dot(u, w, D); /* :108-111 */
 		alpha[it] = a;
-		if (it == 0) { /* :155-159 */
+		if (it == 0) { /* :117-121 */
 			for (int64_t i = 0; i < D; i++)
 				w[i] -= a * u[i];
-		} else { /* :160-173, vector.c:324-380 */
+		} else { /* :122-135, vector.c:324-380 */
 			const double bp = beta[it - 1];
 			const double *up = U + (size_t)(it - 1) * D;
 			for (int64_t i = 0; i < D; i++) {
@@ -324,12 +324,12 @@ int64_t oracle_lanczos(int64_t D, oracle_matvec_fn matvec, void *ctx,
 				}
 			}
 		}
-		const double b = sqrt(dot(w, w, D)); /* :175-176, vector.c:382-404 */
+		const double b = sqrt(dot(w, w, D)); /* :137-138, vector.c:382-404 */
 		beta[it] = b;
-		const double inv = 1.0 / b; /* :179, vector.c:437-459 */
+		const double inv = 1.0 / b; /* :141, vector.c:437-459 */
 		for (int64_t i = 0; i < D; i++)
 			w[i] *= inv;
-		/* lanczos.c:193-207, vector.c:461-476: MGS of u_{it+1} against
+		/* lanczos.c:155-169, vector.c:461-476: MGS of u_{it+1} against
 		 * u_0..u_{it-3}, no renormalization */
 		if (reorth == ORACLE_REORTH_REFERENCE && it > 2) {
 			for (int64_t j = 0; j < it - 2; j++) {
@@ -339,7 +339,7 @@ int64_t oracle_lanczos(int64_t D, oracle_matvec_fn matvec, void *ctx,
 					w[i] -= p * uj[i];
 			}
 		}
-		/* :230-281, target eigenvalue 0 */
+		/* :192-243, target eigenvalue 0 */
 		oracle_tridiagonal_eigenvalues(it + 1, alpha, beta, 1, ev);
 		const double diff = convergence == ORACLE_CONVERGE_EIGENVALUES
 					    ? fabs(ev[0] - previous)

@@ -9,10 +9,10 @@
  *   oracle_matvec             src/Minerva/scheduler/scheduler.c:80-145,
  *                             194-418 (dispatch, diag/off-diag, per-thread
  *                             output replicas src/Minerva/vector_block/
- *                             vector_block.c:274-294,372-382) and the six
+ *                             vector_block.c:47-67,146-155) and the six
  *                             kernels src/Minerva/matrix_vector_multiplication/
  *                             matrix_vector_multiplication.c:7-462
- *   oracle_lanczos            src/Bacchus/lanczos/lanczos.c:98-291 with the
+ *   oracle_lanczos            src/Bacchus/lanczos/lanczos.c:60-253 with the
  *                             block-wise vector ops of src/Bacchus/vector/
  *                             vector.c:240-476 and src/Bacchus/math_tools/
  *                             math_tools.c:13-57

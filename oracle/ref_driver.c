@@ -4,7 +4,7 @@
  * reference tree by oracle/Makefile into oracle/_ref/), bypassing the
  * libconfig settings parser exactly as the reference's own tests do
  * (src/Bacchus/matrix_builder/matrix_builder.c:230-255,
- * src/Bacchus/lanczos/lanczos.c:333-390).
+ * src/Bacchus/lanczos/lanczos.c:295-352).
  *
  *   bacchus_ref matvec  <comb> <Z> <N> <evaluation_order> <index_lists_dir>
  *                       <matrix_dir> <in_dir> <out_dir>

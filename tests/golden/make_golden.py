@@ -7,7 +7,7 @@ in the reference's file formats, and stores:
   x, y_ref          one matvec y = H x through run_matrix_vector_multiplication
                     (src/Minerva/scheduler/scheduler.c:80-145)
   eig_ref_fixed     eigenvalues of T after a fixed number of Lanczos iterations
-                    (no convergence test, src/Bacchus/lanczos/lanczos.c:209-291)
+                    (no convergence test, src/Bacchus/lanczos/lanczos.c:171-253)
   eig_ref_conv      eigenvalues after converging lambda_0 to 1e-7
   krylov_conv       Krylov dimension reached by that run
   digest            checksums of the generated arrays (pins the generator)

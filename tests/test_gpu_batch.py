@@ -7,6 +7,7 @@ import pytest
 from jupiterncsm_amd import bacchus
 from jupiterncsm_amd.inputs import SYSTEMS, SystemConfig, build
 from oracle import oracle as O
+from tests import skip_cases as SK
 
 pytestmark = pytest.mark.gpu
 
@@ -207,41 +208,35 @@ def test_batch_rejects_fp64_operator():
 
 @pytest.mark.parametrize("case", ["N", "P narrow", "P wide", "NP"])
 def test_batched_tiny_element_skip(case):
-    """The batched kernel keeps the reference's 1e-12 element skip for N and P
-    instructions (matrix_vector_multiplication.c:52, :107) and none for NP,
-    on FP32-stored elements: 4 vectors with x of order 1e12, so each skipped
-    product is 0.5 or more, against the oracle in FP64 (FP32 bar 2e-6)."""
+    """The batched kernel skips N and P elements per vector when |s V| <
+    1e-12, s that vector's 2-body scale (matrix_vector_multiplication.c:
+    50-53, :105-108 on the Vulcan-combined element), and nothing for NP, on
+    FP32-stored elements: 4 vectors with scales 1, 0.8, 1.25, 1 and x of
+    order 1e12, so each skipped product is 0.5 or more; against the
+    products written by hand and the oracle in FP64 (FP32 bar 2e-6)."""
     print("case batched tiny element skip", case, flush=True)
     torch = torch_dev()
-    # every product of order 1 (no large element hides a wrong one)
-    V = [5e-13, -5e-13, 2e-12, 1e-12]
-    lst = [(0, 1, 0), (1, 0, 1), (2, 2, 2), (0, 0, 3)]
-    if case == "N":
-        np_, nn, lists, ins, dn = 2, 3, [lst], [1, 1, 0, 0, 0, -1, 0, 0], 4
-    elif case.startswith("P"):
-        np_, nn = 3, (5 if case == "P narrow" else 40)
-        lists, ins, dn = [lst], [2, 1, 0, 0, -1, 0, 0, 0], 1
-    else:
-        np_, nn, dn = 2, 3, 2
-        lists = [[(0, 1, 0), (1, 0, 1), (2, 2, 0)], [(0, 0, 0), (1, 1, 0), (0, 1, 1)]]
-        V = [5e-13, -5e-13, 2e-13, 1e-12]
-        ins = [3, 2, 0, 0, 0, 1, 0, 0]
-    offsets = np.cumsum([0] + [len(e) for e in lists])
-    entries = np.array([t for e in lists for t in e], dtype=np.int32).reshape(-1, 3)
-    arrays = ([np_], [nn], offsets, entries, [0, len(V)], [dn], [2], np.array(V), np.array([ins]))
+    kind = case.split()[0]
+    arrays, np_, nn = SK.problem(kind, wide_p=case == "P wide")
     D = np_ * nn
-    B = 4
+    scales = [1.0, 0.8, 1.25, 1.0]
+    B = len(scales)
     X = 1e12 * (1.0 + np.random.default_rng(6).random((D, B)))
     prob = O.Problem(*arrays)
-    ref = np.stack([O.matvec(prob, X[:, b]) for b in range(B)], axis=1)
+    ref = np.stack([O.matvec(prob, X[:, b], scales=[1.0, scales[b], 1.0]) for b in range(B)],
+                   axis=1)
+    want = np.stack([SK.expected(kind, X[:, b].reshape(np_, nn), scales[b]) for b in range(B)],
+                    axis=1)
+    assert np.abs(ref - want).max() <= 1e-12 * np.abs(want).max()
     with bacchus.Operator.from_arrays(*arrays, precision=bacchus.FP32) as op, \
             bacchus.Batch([op], [B]) as bt:
-        bt.set_scales(0, np.ones((B, 3)))
+        bt.set_scales(0, np.array([[1.0, s, 1.0] for s in scales]))
         xs = [torch.as_tensor(X, dtype=torch.float32, device="cuda").contiguous()]
         ys = [torch.zeros_like(xs[0])]
         bt.matvec(xs, ys)
         torch.cuda.synchronize()
         Y = ys[0].cpu().numpy().astype(np.float64)
-    err = np.abs(Y - ref).max() / np.abs(ref).max()
-    print("batched vs oracle", err, flush=True)
-    assert err <= 2e-6
+    for b in range(B):
+        err = np.abs(Y[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
+        print("batched vs hand-computed, vector", b, err, flush=True)
+        assert err <= 2e-6

@@ -54,6 +54,7 @@ def test_block_lanczos_random_start_matches_dense(b, tol, max_blocks):
     err = np.abs(r.eigenvalues - w) / np.abs(w)
     print("blocks", r.blocks, "errors", err, "residuals", r.residuals, flush=True)
     assert r.blocks < max_blocks
+    assert r.converged
     assert err.max() <= 1e-6
     for k in range(5):
         y = Yh[k]
@@ -79,6 +80,7 @@ def test_block_lanczos_fills_a_tiny_space():
         torch.cuda.synchronize()
     print("blocks", r.blocks, r.eigenvalues, w, flush=True)
     assert r.blocks == 59 // 8
+    assert not r.converged   # CONVERGE_NONE: the block limit ends it
     assert np.all(np.isfinite(r.eigenvalues))
     assert np.all(r.eigenvalues >= w - 1e-5 * np.abs(w).max())
     assert abs(r.eigenvalues[0] - w[0]) <= 1e-6 * abs(w[0])
@@ -162,6 +164,7 @@ def test_batch_and_block_solver_on_an_empty_operator():
         torch.cuda.synchronize()
     print("blocks", r.blocks, r.eigenvalues, r.residuals, flush=True)
     assert r.blocks == 1
+    assert r.converged
     assert np.abs(r.eigenvalues).max() <= 1e-12
     assert np.abs(r.residuals).max() <= 1e-12
 
@@ -187,6 +190,7 @@ def test_block_lanczos_bf16_agrees_with_single_vector_lanczos():
     d = np.abs(r.eigenvalues - s.eigenvalues) / np.abs(s.eigenvalues)
     print("blocks", r.blocks, "block vs single", d, flush=True)
     assert r.blocks < 32
+    assert r.converged
     assert d.max() <= 1e-6
     assert (np.abs(r.eigenvalues - w) / np.abs(w)).max() <= 1e-2
 

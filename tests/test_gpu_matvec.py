@@ -10,6 +10,7 @@ import pytest
 from jupiterncsm_amd import bacchus
 from jupiterncsm_amd.inputs import SYSTEMS, SystemConfig, build
 from oracle import oracle as O
+from tests import skip_cases as SK
 
 pytestmark = pytest.mark.gpu
 
@@ -163,57 +164,25 @@ def test_c3_full_size_matches_oracle():
     assert rel_err(h32, ref) <= 2e-6
 
 
-def _one_block(np_, nn, lists, elements, instruction, dn):
-    """Arrays of a one-block problem: lists = [(in, out, m), ...] per list."""
-    offsets = np.cumsum([0] + [len(e) for e in lists])
-    entries = np.array([t for e in lists for t in e], dtype=np.int32).reshape(-1, 3)
-    return ([np_], [nn], offsets, entries, [0, len(elements)], [dn], [2],
-            np.asarray(elements, dtype=np.float64), np.array([instruction]))
-
-
+@pytest.mark.parametrize("scale", SK.SCALES)
 @pytest.mark.parametrize("case", ["N", "P narrow", "P wide", "NP"])
-def test_tiny_element_skip(case):
-    """N and P instructions skip elements with |V| < 1e-12, NP instructions
-    do not (matrix_vector_multiplication.c:52, :107 against :134-210); an
-    element of exactly 1e-12 counts.  x of order 1e12 makes each skipped
-    product 0.5 or more, far above FP64 rounding: the device result must
-    equal the product computed by hand, and the oracle's."""
-    print("case tiny element skip", case, flush=True)
-    # every product of order 1 (no large element hides a wrong one)
-    V = [5e-13, -5e-13, 2e-12, 1e-12]
-    lst = [(0, 1, 0), (1, 0, 1), (2, 2, 2), (0, 0, 3)]
-    if case == "N":
-        np_, nn = 2, 3
-        arrays = _one_block(np_, nn, [lst], V, [1, 1, 0, 0, 0, -1, 0, 0], 4)
-    elif case.startswith("P"):
-        np_, nn = 3, (5 if case == "P narrow" else 40)
-        arrays = _one_block(np_, nn, [lst], V, [2, 1, 0, 0, -1, 0, 0, 0], 1)
-    else:
-        np_, nn = 2, 3
-        neutrons = [(0, 1, 0), (1, 0, 1), (2, 2, 0)]
-        protons = [(0, 0, 0), (1, 1, 0), (0, 1, 1)]
-        V = [5e-13, -5e-13, 2e-13, 1e-12]   # m_n + 2 m_p
-        arrays = _one_block(np_, nn, [neutrons, protons], V, [3, 2, 0, 0, 0, 1, 0, 0], 2)
+def test_tiny_element_skip(case, scale):
+    """N and P instructions skip elements with |s V| < 1e-12 (s the force
+    scale: the reference tests the Vulcan-combined element it reads,
+    matrix_vector_multiplication.c:50-53, :105-108), NP instructions skip
+    nothing (:134-210); exactly 1e-12 counts.  Elements straddle 1e-12 / s
+    and x is of order 1e12, so each skipped product is 0.5 or more, far
+    above FP64 rounding: the device result must equal the products computed
+    by hand, and the oracle's."""
+    print("case tiny element skip", case, "scale", scale, flush=True)
+    arrays, np_, nn = SK.problem(case.split()[0], wide_p=case == "P wide")
     rng = np.random.default_rng(5)
-    x = 1e12 * (1.0 + rng.random(np_ * nn))
-    X = x.reshape(np_, nn)
-    Y = np.zeros((np_, nn))
-    if case == "N":
-        for i, o, m in lst:
-            if abs(V[m]) >= 1e-12:
-                Y[:, o] += V[m] * X[:, i]
-    elif case.startswith("P"):
-        for i, o, m in lst:
-            if abs(V[m]) >= 1e-12:
-                Y[o, :] += V[m] * X[i, :]
-    else:
-        for ni_, no, mn in neutrons:
-            for pi, po, mp in protons:
-                Y[po, no] += V[mn + 2 * mp] * X[pi, ni_]
-    expect = Y.ravel()
+    X = 1e12 * (1.0 + rng.random((np_, nn)))
+    expect = SK.expected(case.split()[0], X, scale)
     with bacchus.Operator.from_arrays(*arrays) as op:
-        y = gpu_matvec(op, x)
-    ref = O.matvec(O.Problem(*arrays), x)
+        op.set_scales(1.0, scale, 1.0)
+        y = gpu_matvec(op, X.ravel())
+    ref = O.matvec(O.Problem(*arrays), X.ravel(), scales=[1.0, scale, 1.0])
     print("device", y, "expected", expect, flush=True)
     assert rel_err(y, expect) <= FP64_TOL
     assert rel_err(ref, expect) <= FP64_TOL
