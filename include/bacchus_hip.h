@@ -203,6 +203,12 @@ typedef struct bacchus_hip_lanczos_settings {
 	int32_t num_ritz_vectors; /* Ritz vectors written to ritz_vector
 				     (0 = 1)                             */
 	int32_t pad;
+	double *krylov_basis;     /* NULL, or host memory for
+				     min(D, max_num_iterations) x D doubles
+				     receiving the Krylov vectors u_0 ..
+				     u_{k-1} of the solve (the basis
+				     get_eigensystem combines, lanczos.c:
+				     255-265)                            */
 } bacchus_hip_lanczos_settings_t;
 
 typedef struct bacchus_hip_lanczos_result {

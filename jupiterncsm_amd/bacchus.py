@@ -79,7 +79,7 @@ class _LanczosSettings(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("start_vector", ctypes.c_void_p),
         ("target_eigenvalues", ctypes.c_void_p), ("check_every", ctypes.c_int32),
         ("target_eigenvalue", ctypes.c_int32), ("num_ritz_vectors", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("pad", ctypes.c_int32), ("krylov_basis", ctypes.c_void_p),
     ]
 
 
@@ -386,7 +386,7 @@ def diagonalize(matrix: Operator, max_num_iterations: int, *, num_eigenvalues=5,
         int(num_eigenvalues), float(tolerance), int(seed),
         ctypes.c_void_p(start_vector.data_ptr()) if start_vector is not None else None,
         _ptr(targets) if targets is not None else None, int(check_every),
-        int(target_eigenvalue), int(num_ritz_vectors), 0)
+        int(target_eigenvalue), int(num_ritz_vectors), 0, None)
     if ritz_vector is not None and ritz_vector.numel() < int(num_ritz_vectors) * matrix.dimension:
         raise BacchusError(f"ritz_vector must hold {num_ritz_vectors} x {matrix.dimension} doubles")
     kmax = max(1, min(matrix.dimension, int(max_num_iterations)))

@@ -540,6 +540,12 @@ extern "C" int bacchus_hip_lanczos(bacchus_hip_operator_t *op,
 			}
 		}
 	}
+	if (!rc && st->krylov_basis && dim > 0) {
+		if (hipMemcpyAsync(st->krylov_basis, U, (size_t)dim * D * sizeof(double),
+				   hipMemcpyDeviceToHost, L.s) != hipSuccess ||
+		    hipStreamSynchronize(L.s) != hipSuccess)
+			rc = BACCHUS_FAIL("Krylov basis copy failed");
+	}
 	for (int64_t k = 0; k < dim && !rc; k++) {
 		float ms = 0;
 		if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess)

@@ -10,6 +10,10 @@ in the reference's file formats, and stores:
                     (no convergence test, src/Bacchus/lanczos/lanczos.c:171-253)
   eig_ref_conv      eigenvalues after converging lambda_0 to 1e-7
   krylov_conv       Krylov dimension reached by that run
+  eigvec_ref_conv   eigenvector_1 and eigenvector_2 written by that run the way
+                    src/Bacchus/programs/bacchus.c:70-87 writes them
+  eig_ref_vec       eigenvalues after the eigenvector-amplitude criterion
+                    (lanczos.c:276-287) at 1e-5, krylov_vec its Krylov dimension
   digest            checksums of the generated arrays (pins the generator)
 
 and, from the compiled Mars program (oracle/_ref/mars_ref,
@@ -91,13 +95,23 @@ def main():
             run(["lanczos", *comb, os.path.join(work, "k1"), str(iters), "1e-12", "2",
                  os.path.join(work, "e1")])
             eig_fixed = np.loadtxt(os.path.join(work, "e1"), ndmin=1)
+            os.makedirs(os.path.join(work, "ev"))
             run(["lanczos", *comb, os.path.join(work, "k2"), "200", "1e-7", "0",
-                 os.path.join(work, "e2")])
+                 os.path.join(work, "e2"), os.path.join(work, "ev"), "1"])
             eig_conv = np.loadtxt(os.path.join(work, "e2"), ndmin=1)
+            # eigenvector_1, eigenvector_2 of that run (bacchus.c:70-87)
+            eigvec = np.stack([read_vector(inp, os.path.join(work, "ev", f"eigenvector_{i + 1}"))
+                               for i in range(2)])
+            # the eigenvector-amplitude criterion (lanczos.c:276-287)
+            run(["lanczos", *comb, os.path.join(work, "k3"), "200", "1e-5", "1",
+                 os.path.join(work, "e3")])
+            eig_vec = np.loadtxt(os.path.join(work, "e3"), ndmin=1)
             ints, floats = digest(inp)
             np.savez_compressed(os.path.join(out_dir, f"{name}.npz"), x=x, y_ref=y,
                                 eig_ref_fixed=eig_fixed, iterations_fixed=iters,
                                 eig_ref_conv=eig_conv, krylov_conv=len(eig_conv),
+                                eigvec_ref_conv=eigvec, eig_ref_vec=eig_vec,
+                                krylov_vec=len(eig_vec),
                                 digest_int=ints, digest_float=floats)
             print(name, "D", inp.dimension, "|y|", np.abs(y).max(), "lowest", eig_fixed[:3],
                   "conv K", len(eig_conv))
