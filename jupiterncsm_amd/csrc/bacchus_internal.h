@@ -98,6 +98,8 @@ struct PlanParams {
 	int64_t tile_elems = kTileMaxElems;  // LDS words per tile (rows*nn*bt)
 	uint64_t task_mults = kTaskMults;    // multiplications x bt per task
 	int xcds = 0;               // > 1: group output blocks per XCD
+	bool skip_np = false;       // leave NP instructions out (np_kernel
+				    // applies them)
 };
 struct Plan {
 	std::vector<TileWork> tiles;
@@ -106,6 +108,65 @@ struct Plan {
 	std::vector<int> bt;        // vectors of every tile
 	int64_t max_tile_elems = 1;
 };
+
+// --- the neutron-proton (NP) path: element-row-major units ---
+//
+// An NP virtual instruction (one direction of an NP instruction, in the
+// space where its lane list is the longer list) is
+//   y[s_out(j)][l_out(i)] += sign_j sign_i W[a_i][b_j] x[s_in(j)][l_in(i)]
+// over the row-list entries j (compact index b_j) and lane-list entries i
+// (compact index a_i).  Its lane list is stored sliced ("SELL"): the output
+// states with entries, sorted by entry count, q lanes per state, 64 lanes
+// per slice, entry t q + u of a state at [slice][t][lane]; the element rows
+// are expanded into that order per row-list compact index b (W_b[pos] =
+// sign_i W[a_i][b], zero at the padding), so a wave streams W_b and the
+// input indices with 16- and 4-byte coalesced loads.  The row-list entries
+// are grouped by b: one W_b value in a register serves every j with
+// compact index b, each j adding into its own output row.
+struct NpInstr {
+	int64_t x_off;      // input block offset
+	int64_t w_off;      // first element of W_0 (row stride sell_len)
+	uint32_t sell_len;  // padded SELL length of the lane list
+	uint32_t slice0;    // first slice (global index into the slice table)
+	uint32_t nslices;
+	uint32_t bbase;     // row-list entries of compact index b:
+			    // [bptr[bbase + b], bptr[bbase + b + 1])
+	int32_t nl_in;      // lane-side states of the input block (x row stride)
+	int16_t force;
+	int16_t pad;
+	uint32_t sell_base; // first position of the lane list's SELL copy
+};
+// A slice: `width` lanes (its states x q lanes each, q = 1 << lq chosen
+// per slice to waste the fewest positions), `steps` positions per lane,
+// position t width + lane.
+struct NpSlice {
+	uint32_t off;       // first SELL position of the slice, from the
+			    // lane list's sell_base (= its element column)
+	uint32_t steps;     // entries per lane
+	uint32_t row_base;  // row table entry of lane 0 (64 per slice)
+	uint32_t shape;     // width | lq << 16
+};
+// one wave's work: instruction vi, compact indices [b0, b1), slices
+// [k0, k1) of its lane list
+struct NpUnit {
+	int32_t vi;
+	uint32_t b0, b1;
+	uint32_t k0, k1;
+	uint32_t pad;
+};
+// one workgroup: a whole output block (rows x cols of its space) in LDS
+// and the units that write it
+struct NpTile {
+	int64_t y_off;
+	int32_t rows;
+	int32_t cols;
+	int32_t space;
+	int32_t u_begin;
+	int32_t u_end;
+	int32_t pad;
+};
+constexpr int kNpThreads = 512;
+constexpr int kNpChunk = 4;  // row-list entries (same b) per pass
 
 // Shape of a batch: operators, vectors of operator 0, its dimension.
 int batch_shape(const bacchus_hip_batch_t *batch, int32_t *num_ops, int32_t *num_vectors,
@@ -156,8 +217,24 @@ struct bacchus_hip_operator {
 	std::vector<uint32_t> h_rowptr;
 	std::vector<bacchus::TInstr> h_tinstr;
 	std::vector<int32_t> blk_vi;
+	std::vector<int32_t> blk_np;    // first NP virtual instruction of each
+					// slot (they come last in a slot)
 	double mults = 0;
 	int num_cu = 256;
+	// NP path (np.hip)
+	bool np_path = false;           // NP instructions run in np_kernel
+	void *d_np_w = nullptr;         // expanded element rows (storage type)
+	uint32_t *d_np_in = nullptr;    // SELL input states (all lane lists)
+	int32_t *d_np_row = nullptr;    // SELL output state of every lane
+	bacchus::NpSlice *d_np_slices = nullptr;
+	uint2 *d_np_j = nullptr;        // row-list entries {s_in, s_out|sign}
+	uint32_t *d_np_bptr = nullptr;  // per (row list, direction): cS + 1
+	bacchus::NpInstr *d_np_instr = nullptr;
+	bacchus::NpUnit *d_np_units = nullptr;
+	bacchus::NpTile *d_np_tiles = nullptr;
+	int64_t np_num_tiles = 0;
+	size_t np_lds_bytes = 0;
+	int64_t np_elements = 0;        // stored expanded elements
 	// statistics
 	size_t device_bytes = 0;        // device memory the operator holds
 	double algorithmic_bytes = 0;
@@ -169,6 +246,28 @@ struct bacchus_hip_operator {
 
 namespace bacchus {
 int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out);
+
+// What the NP tables are built from (bacchus_hip_operator_create's host
+// tables): the problem, every list's compact element indices, which
+// instructions run in the transposed space, and the CSR copies of every
+// used list (entries {other index, compact | sign}, rows by output at
+// rows_by_out[l], by input at rows_by_in[l], both absolute in rowptr).
+struct NpContext {
+	const bacchus_hip_problem_t *p;
+	const std::vector<std::vector<uint32_t>> *mvals;
+	const std::vector<char> *swapped;
+	const std::vector<uint32_t> *rowptr;
+	const std::vector<uint2> *entries;
+	const std::vector<int64_t> *rows_by_out, *rows_by_in;
+	const std::vector<int64_t> *n_in, *n_out;  // sub-basis sizes per list
+};
+// Builds and uploads the NP tables; sets op->np_path.
+int np_build(bacchus_hip_operator *op, const NpContext &c);
+// Applies every NP instruction: y (and yT) += H_NP x, atomically.
+int np_launch(const bacchus_hip_operator *op, const void *x, const void *xT, double *y,
+	      double s1, double s2, double s3, hipStream_t s);
+// Frees the NP tables.
+void np_free(bacchus_hip_operator *op);
 
 constexpr int kTransposeSplits = 8;
 

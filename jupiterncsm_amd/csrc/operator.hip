@@ -580,7 +580,7 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 	std::vector<int> tile_bt;
 	for (int64_t slot = 0; slot < 2 * nb; slot++) {
 		const int32_t first = op->blk_vi[slot];
-		int32_t last = op->blk_vi[slot + 1];
+		int32_t last = pp.skip_np ? op->blk_np[slot] : op->blk_vi[slot + 1];
 		if (first == last)
 			continue;
 		const int64_t b = slot % nb;
@@ -780,6 +780,7 @@ void bacchus_hip_operator_free(bacchus_hip_operator_t *op)
 	(void)hipFree(op->d_blk_off);
 	(void)hipFree(op->d_blk_np);
 	(void)hipFree(op->d_blk_nn);
+	np_free(op);
 	delete op;
 }
 
@@ -1400,11 +1401,17 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	// --- host copies for the planner; virtual instructions of a block
 	// sorted by element sub-block so neighbouring tasks share elements ---
 	op->blk_vi.assign(2 * nb + 1, 0);
+	op->blk_np.assign(2 * nb, 0);
 	for (int64_t b = 0; b < 2 * nb; b++) {
 		auto &vs = per_block[b];
-		// by element sub-block
-		std::stable_sort(vs.begin(), vs.end(),
-				 [](const TInstr &u, const TInstr &w) { return u.v_off < w.v_off; });
+		// N and P first, NP last (np_kernel can take those), each by
+		// element sub-block
+		std::stable_sort(vs.begin(), vs.end(), [](const TInstr &u, const TInstr &w) {
+			return std::make_pair(u.type == 3, u.v_off) < std::make_pair(w.type == 3, w.v_off);
+		});
+		op->blk_np[b] = (int32_t)(op->h_tinstr.size() +
+					  std::count_if(vs.begin(), vs.end(),
+							[](const TInstr &t) { return t.type != 3; }));
 		op->h_tinstr.insert(op->h_tinstr.end(), vs.begin(), vs.end());
 		op->blk_vi[b + 1] = (int32_t)op->h_tinstr.size();
 		if (b >= nb && !vs.empty())
@@ -1419,6 +1426,22 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		    hipGetDeviceProperties(&prop, dev) == hipSuccess)
 			op->num_cu = prop.multiProcessorCount;
 	}
+	// NP instructions in np_kernel (element-row-major units over whole
+	// output blocks); BACCHUS_HIP_NP=0 keeps them in the tile kernel
+	{
+		const char *e = getenv("BACCHUS_HIP_NP");
+		if (!e || atoi(e) != 0) {
+			std::vector<int64_t> n_in(nl, 0), n_out(nl, 0);
+			for (int64_t l = 0; l < nl; l++) {
+				n_in[l] = use[l].n_in;
+				n_out[l] = use[l].n_out;
+			}
+			NpContext nc{p, &mvals, &swapped, &rowptr, &entries, &rows_by_out, &rows_by_in,
+				     &n_in, &n_out};
+			if (int rc = np_build(op, nc))
+				return fail(rc);
+		}
+	}
 	// ~64 tiles per CU: small tiles keep every CU busy to the end
 	PlanParams pp;
 	pp.target = std::max(2.0e4, mults / (op->num_cu * 64.0));
@@ -1426,6 +1449,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		pp.target = std::max(1.0e3, atof(tm));
 	pp.bt.assign(2 * nb, 1);
 	pp.xcds = kXcds;
+	pp.skip_np = op->np_path;
 	if (const char *xe = getenv("BACCHUS_HIP_XCDS"))
 		pp.xcds = atoi(xe);
 	Plan plan;
@@ -1548,6 +1572,8 @@ int bacchus_hip_operator_matvec(bacchus_hip_operator_t *op, const double *x,
 			hipLaunchKernelGGL(transpose_blocks<double>, tgrid, dim3(threads), 0, s, x, xT,
 					   op->d_blk_off, op->d_blk_np, op->d_blk_nn, 1);
 		launch<double>(op, (const double *)op->d_v, x, xT, y, s1, s2, s3, s);
+		if (int rc = np_launch(op, x, xT, y, s1, s2, s3, s))
+			return rc;
 	} else {
 		float *xT = (float *)op->d_xT;
 		hipLaunchKernelGGL(to_float_kernel, dim3(blocks), dim3(threads), 0, s, x, op->d_xf, n);
@@ -1558,6 +1584,8 @@ int bacchus_hip_operator_matvec(bacchus_hip_operator_t *op, const double *x,
 			launch<float>(op, (const float *)op->d_v, op->d_xf, xT, y, s1, s2, s3, s);
 		else
 			launch<uint16_t>(op, (const uint16_t *)op->d_v, op->d_xf, xT, y, s1, s2, s3, s);
+		if (int rc = np_launch(op, op->d_xf, xT, y, s1, s2, s3, s))
+			return rc;
 	}
 	if (op->has_transposed)
 		hipLaunchKernelGGL(add_transposed_blocks<double>, tgrid, dim3(threads), 0, s, y,
