@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -133,7 +134,8 @@ struct NpInstr {
 			    // [bptr[bbase + b], bptr[bbase + b + 1])
 	int32_t nl_in;      // lane-side states of the input block (x row stride)
 	int16_t force;
-	int16_t pad;
+	int16_t stage;      // x rows staged in LDS per pass (0: x gathered
+			    // from global memory, kNpChunk rows per pass)
 	uint32_t sell_base; // first position of the lane list's SELL copy
 };
 // A slice: `width` lanes (its states x q lanes each, q = 1 << lq chosen
@@ -152,7 +154,9 @@ struct NpUnit {
 	int32_t vi;
 	uint32_t b0, b1;
 	uint32_t k0, k1;
-	uint32_t pad;
+	uint32_t jsub;  // 0: every entry of [b0, b1); else b1 = b0 + 1 and the
+			// entries first .. first + count of b0 (first << 16 |
+			// count, count <= 64)
 };
 // one workgroup: a whole output block (rows x cols of its space) in LDS
 // and the units that write it
@@ -167,6 +171,10 @@ struct NpTile {
 };
 constexpr int kNpThreads = 512;
 constexpr int kNpChunk = 4;  // row-list entries (same b) per pass
+constexpr int kNpSteps = 8;  // lane steps per round of loads
+constexpr uint32_t kNpUnitSlices = 64;  // slices (and compact indices + 1)
+					// of a unit at most
+constexpr int kNpStage = 1024;  // x values of a wave's LDS staging area
 
 // Shape of a batch: operators, vectors of operator 0, its dimension.
 int batch_shape(const bacchus_hip_batch_t *batch, int32_t *num_ops, int32_t *num_vectors,
@@ -234,6 +242,7 @@ struct bacchus_hip_operator {
 	bacchus::NpTile *d_np_tiles = nullptr;
 	int64_t np_num_tiles = 0;
 	size_t np_lds_bytes = 0;
+	int32_t np_block_words = 0;     // LDS words of the largest block
 	int64_t np_elements = 0;        // stored expanded elements
 	// statistics
 	size_t device_bytes = 0;        // device memory the operator holds

@@ -34,6 +34,7 @@
 #include <numeric>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 
 namespace bacchus {
 namespace {
@@ -61,9 +62,13 @@ template <> struct NpElem<uint16_t> {
 	}
 };
 
-// One workgroup per tile: the output block in LDS (FP64 sums), waves take
-// the tile's units in turn; every unit is one instruction direction, a
-// range of row-list compact indices b and a range of lane-list slices.
+// One workgroup per tile: the output block in LDS (FP64 sums) followed by
+// one x staging area of kNpStage words per wave; waves take the tile's
+// units in turn.  A unit is one instruction direction, a range of row-list
+// compact indices b and a range of lane-list slices.  For instructions
+// whose x rows fit the staging area (I.stage rows per pass), a pass copies
+// its rows there once and every slice reads x from LDS; the others gather
+// x from global memory, kNpChunk rows per pass.
 template <typename VT>
 __global__ __launch_bounds__(kNpThreads) void np_kernel(
 	const NpTile *__restrict__ tiles, const NpUnit *__restrict__ units,
@@ -72,91 +77,185 @@ __global__ __launch_bounds__(kNpThreads) void np_kernel(
 	const uint2 *__restrict__ jl, const uint32_t *__restrict__ bptr,
 	const VT *__restrict__ W, const typename NpElem<VT>::x_t *__restrict__ x_normal,
 	const typename NpElem<VT>::x_t *__restrict__ x_transposed, double *__restrict__ y_normal,
-	double *__restrict__ y_transposed, double s1, double s2, double s3)
+	double *__restrict__ y_transposed, int32_t block_words, double s1, double s2, double s3)
 {
 	using acc_t = typename NpElem<VT>::acc_t;
+	using x_t = typename NpElem<VT>::x_t;
 	extern __shared__ double blk[];
 	const NpTile T = tiles[blockIdx.x];
 	const int words = T.rows * T.cols;
 	for (int t = threadIdx.x; t < words; t += blockDim.x)
 		blk[t] = 0.0;
 	__syncthreads();
-	const auto *__restrict__ xs = T.space ? x_transposed : x_normal;
+	const x_t *__restrict__ xs = T.space ? x_transposed : x_normal;
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int nwaves = blockDim.x >> 6;
+	// this wave's staging area (x values in their own type)
+	x_t *__restrict__ stage =
+		reinterpret_cast<x_t *>(blk + block_words) + (size_t)wave * kNpStage;
 	for (int32_t ui = T.u_begin + wave; ui < T.u_end; ui += nwaves) {
 		const NpUnit U = units[ui];
 		const NpInstr I = ins[U.vi];
 		const acc_t sc = acc_t(I.force == 1 ? s1 : (I.force == 2 ? s2 : s3));
-		const auto *__restrict__ xb = xs + I.x_off;
-		for (uint32_t b = U.b0; b < U.b1; b++) {
-			const uint32_t j0 = bptr[I.bbase + b], j1 = bptr[I.bbase + b + 1];
+		const x_t *__restrict__ xb = xs + I.x_off;
+		const uint32_t nl = (uint32_t)I.nl_in;
+		const int rows_per_pass = I.stage ? I.stage : kNpChunk;
+		// the unit's metadata in lane registers (the builder keeps every
+		// unit within 64 slices, 63 compact indices and 64 row-list
+		// entries), read back with readlane: no dependent loads per slice
+		// or pass
+		const uint32_t nsl = U.k1 - U.k0, nb = U.b1 - U.b0;
+		uint4 sl = make_uint4(0, 0, 0, 0);
+		if ((uint32_t)lane < nsl) {
+			const NpSlice q = slices[I.slice0 + U.k0 + lane];
+			sl = make_uint4(q.off, q.steps, q.row_base, q.shape);
+		}
+		const uint32_t bp = (uint32_t)lane <= nb ? bptr[I.bbase + U.b0 + lane] : 0u;
+		uint32_t jfirst = __builtin_amdgcn_readfirstlane(bp);
+		uint32_t jend = __builtin_amdgcn_readlane(bp, nb);
+		if (U.jsub) {
+			// one compact index with more than 64 entries: this unit's
+			// part of them
+			jfirst += U.jsub >> 16;
+			jend = jfirst + (U.jsub & 0xFFFFu);
+		}
+		const uint2 je = (uint32_t)lane < jend - jfirst ? jl[jfirst + lane] : make_uint2(0u, 0u);
+		for (uint32_t bi = 0; bi < nb; bi++) {
+			const uint32_t b = U.b0 + bi;
+			const uint32_t j0 = U.jsub ? 0u : __builtin_amdgcn_readlane(bp, bi) - jfirst;
+			const uint32_t j1 = U.jsub ? jend - jfirst
+						   : __builtin_amdgcn_readlane(bp, bi + 1) - jfirst;
 			const VT *__restrict__ wb = W + I.w_off + (size_t)b * I.sell_len;
-			for (uint32_t jc = j0; jc < j1; jc += kNpChunk) {
-				const int n = (int)min((uint32_t)kNpChunk, j1 - jc);
+			for (uint32_t jc = j0; jc < j1; jc += (uint32_t)rows_per_pass) {
+				const int n = (int)min((uint32_t)rows_per_pass, j1 - jc);
 				// row-list entries of this pass (wave-uniform)
 				uint32_t xrow[kNpChunk], orow[kNpChunk];
 				acc_t sg[kNpChunk];
 #pragma unroll
 				for (int c = 0; c < kNpChunk; c++) {
-					const uint2 e = jl[jc + (c < n ? c : 0)];
-					xrow[c] = (uint32_t)I.nl_in * __builtin_amdgcn_readfirstlane(e.x);
-					const uint32_t so = __builtin_amdgcn_readfirstlane(e.y);
+					const uint32_t at = jc + (uint32_t)(c < n ? c : 0);
+					xrow[c] = nl * __builtin_amdgcn_readlane(je.x, at);
+					const uint32_t so = __builtin_amdgcn_readlane(je.y, at);
 					orow[c] = (so & kMask) * (uint32_t)T.cols;
 					sg[c] = (so & 0x80000000u) ? -sc : sc;
 				}
-				for (uint32_t k = U.k0; k < U.k1; k++) {
-					const NpSlice S = slices[I.slice0 + k];
-					const uint32_t width = S.shape & 0xFFFFu;
-					const int q = 1 << (S.shape >> 16);
-					acc_t acc[kNpChunk];
+				if (I.stage) {
+					// the pass's x rows into the staging area; the wave's
+					// earlier reads of it are complete (their values fed
+					// the sums already flushed)
+					__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+					// all loads first (one wait), then the LDS writes:
+					// position p of the area is row p / nl, column p % nl
+					{
+						const uint32_t total = (uint32_t)n * nl;
+						x_t tmp[kNpStage / 64];
 #pragma unroll
-					for (int c = 0; c < kNpChunk; c++)
-						acc[c] = 0;
-					if ((uint32_t)lane < width) {
-						const uint32_t *__restrict__ ein = sell_in + I.sell_base + S.off + lane;
-						const VT *__restrict__ w = wb + S.off + lane;
-						if (n == kNpChunk) {
-							for (uint32_t t = 0; t < S.steps; t++) {
-								const uint32_t p = t * width;
-								const uint32_t e = ein[p];
-								const acc_t v = NpElem<VT>::load(w, p);
-								acc_t xv[kNpChunk];
-#pragma unroll
-								for (int c = 0; c < kNpChunk; c++)
-									xv[c] = acc_t(xb[xrow[c] + e]);
-#pragma unroll
-								for (int c = 0; c < kNpChunk; c++)
-									acc[c] += v * xv[c];
-							}
-						} else {
-							for (uint32_t t = 0; t < S.steps; t++) {
-								const uint32_t p = t * width;
-								const uint32_t e = ein[p];
-								const acc_t v = NpElem<VT>::load(w, p);
-								acc_t xv[kNpChunk];
-#pragma unroll
-								for (int c = 0; c < kNpChunk; c++)
-									xv[c] = c < n ? acc_t(xb[xrow[c] + e]) : acc_t(0);
-#pragma unroll
-								for (int c = 0; c < kNpChunk; c++)
-									acc[c] += v * xv[c];
+						for (int k = 0; k < kNpStage / 64; k++) {
+							const uint32_t p = (uint32_t)lane + 64u * (uint32_t)k;
+							tmp[k] = x_t(0);
+							if (p < total) {
+								const uint32_t c = p / nl, i = p - c * nl;
+								const uint32_t r = c == 0 ? xrow[0] : c == 1 ? xrow[1]
+										 : c == 2 ? xrow[2] : xrow[3];
+								tmp[k] = xb[r + i];
 							}
 						}
+#pragma unroll
+						for (int k = 0; k < kNpStage / 64; k++) {
+							const uint32_t p = (uint32_t)lane + 64u * (uint32_t)k;
+							if (p < total)
+								stage[p] = tmp[k];
+						}
 					}
-					// a state's q lanes hold parts of its sum (width is a
-					// multiple of q, so a group is all in or all out)
-					for (int o = q >> 1; o > 0; o >>= 1)
+					__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+					// rows past n read row 0 (their sums are never used)
+#pragma unroll
+					for (int c = 0; c < kNpChunk; c++)
+						xrow[c] = (uint32_t)(c < n ? c : 0) * nl;
+				}
+				// the pass's slices as one pipeline of rounds (kNpSteps
+				// steps of one slice each): the index, element and row
+				// loads of the next round are issued before the products
+				// of the current one, across slice boundaries
+				const uint32_t *__restrict__ ein0 = sell_in + I.sell_base;
+				const x_t *__restrict__ xsrc = I.stage ? stage : xb;
+				auto issue = [&](uint32_t k, uint32_t t, uint32_t (&e)[kNpSteps],
+						 acc_t (&v)[kNpSteps]) {
+					const uint32_t off = __builtin_amdgcn_readlane(sl.x, k);
+					const uint32_t steps = __builtin_amdgcn_readlane(sl.y, k);
+					const uint32_t width = __builtin_amdgcn_readlane(sl.w, k) & 0xFFFFu;
+#pragma unroll
+					for (int u = 0; u < kNpSteps; u++) {
+						const bool ok = (uint32_t)lane < width && t + (uint32_t)u < steps;
+						const uint32_t p = off + (t + (uint32_t)u) * width + (uint32_t)lane;
+						e[u] = ok ? ein0[p] : 0u;
+						v[u] = ok ? NpElem<VT>::load(wb, p) : acc_t(0);
+					}
+				};
+				auto row_of = [&](uint32_t k) {
+					return sell_row[__builtin_amdgcn_readlane(sl.z, k) + (uint32_t)lane];
+				};
+				uint32_t ec[kNpSteps], en[kNpSteps];
+				acc_t vc[kNpSteps], vn[kNpSteps];
+				acc_t acc[kNpChunk];
+#pragma unroll
+				for (int c = 0; c < kNpChunk; c++)
+					acc[c] = 0;
+				uint32_t k = 0, t = 0;
+				issue(0, 0, ec, vc);
+				int32_t row_c = row_of(0), row_n = -1;
+				for (;;) {
+					const uint32_t steps = __builtin_amdgcn_readlane(sl.y, k);
+					const bool same = t + kNpSteps < steps;
+					const uint32_t kn = same ? k : k + 1, tn = same ? t + kNpSteps : 0u;
+					const bool more = kn < nsl;
+					if (more) {
+						issue(kn, tn, en, vn);
+						if (!same)
+							row_n = row_of(kn);
+					}
+					// products of the current round (missing steps have zero
+					// elements and read input 0)
+#pragma unroll
+					for (int u = 0; u < kNpSteps; u++) {
+						acc_t xv[kNpChunk];
 #pragma unroll
 						for (int c = 0; c < kNpChunk; c++)
-							acc[c] += __shfl_xor(acc[c], o, 64);
-					const int32_t row = sell_row[S.row_base + lane];
-					if ((lane & (q - 1)) == 0 && row >= 0) {
+							xv[c] = c < n ? acc_t(xsrc[xrow[c] + ec[u]]) : acc_t(0);
 #pragma unroll
 						for (int c = 0; c < kNpChunk; c++)
-							if (c < n)
-								atomicAdd(&blk[orow[c] + (uint32_t)row], (double)(sg[c] * acc[c]));
+							acc[c] += vc[u] * xv[c];
+					}
+					if (!same) {
+						// slice k done: a state's q lanes hold parts of its
+						// sum (width is a multiple of q, so a group is all in
+						// or all out)
+						const int q = 1 << (__builtin_amdgcn_readlane(sl.w, k) >> 16);
+						for (int o = q >> 1; o > 0; o >>= 1)
+#pragma unroll
+							for (int c = 0; c < kNpChunk; c++)
+								acc[c] += __shfl_xor(acc[c], o, 64);
+						if ((lane & (q - 1)) == 0 && row_c >= 0) {
+#pragma unroll
+							for (int c = 0; c < kNpChunk; c++)
+								if (c < n)
+									atomicAdd(&blk[orow[c] + (uint32_t)row_c],
+										  (double)(sg[c] * acc[c]));
+						}
+#pragma unroll
+						for (int c = 0; c < kNpChunk; c++)
+							acc[c] = 0;
+						row_c = row_n;
+					}
+					if (!more)
+						break;
+					k = kn;
+					t = tn;
+#pragma unroll
+					for (int u = 0; u < kNpSteps; u++) {
+						ec[u] = en[u];
+						vc[u] = vn[u];
 					}
 				}
 			}
@@ -512,6 +611,19 @@ int np_build(bacchus_hip_operator *op, const NpContext &c)
 		return e ? std::max(1024.0, atof(e)) : 65536.0;
 	}();
 	double total_cost = 0;
+	// x rows staged in LDS (BACCHUS_HIP_NP_STAGE=0: gathered from global
+	// memory through the caches)
+	const bool stage_on = [] {
+		const char *e = getenv("BACCHUS_HIP_NP_STAGE");
+		return !e || atoi(e) != 0;
+	}();
+	// row-list entries of a unit at most (lane registers; smaller values
+	// exercise the split of a compact index's entries)
+	const uint32_t jmax = [] {
+		const char *e = getenv("BACCHUS_HIP_NP_JMAX");
+		const int v = e ? atoi(e) : 64;
+		return (uint32_t)std::min(64, std::max(1, v));
+	}();
 	for (size_t k = 0; k < virts.size(); k++) {
 		const Virt &v = virts[k];
 		const bacchus_hip_instruction_t &I = p->instructions[v.i];
@@ -528,6 +640,13 @@ int np_build(bacchus_hip_operator *op, const NpContext &c)
 		n.nl_in = (int32_t)(v.space ? op->block_np[v.bin] : op->block_nn[v.bin]);
 		const int force = p->matrix_force ? p->matrix_force[I.matrix_block] : 2;
 		n.force = (int16_t)std::min(3, std::max(1, force));
+		// x rows staged in LDS when they fit and the lane list is long
+		// enough to pay for copying them
+		{
+			const int fit = kNpStage / std::max(1, n.nl_in);
+			n.stage = (int16_t)((stage_on && fit >= 1 && S.len >= (uint32_t)n.nl_in / 2)
+						    ? std::min(kNpChunk, fit) : 0);
+		}
 		// cost of one b: its entries x the lane-steps of every slice
 		double steps = 0;
 		for (uint32_t s = 0; s < S.nslices; s++)
@@ -541,28 +660,41 @@ int np_build(bacchus_hip_operator *op, const NpContext &c)
 				continue;
 			}
 			const double cb = J.count[b] * steps;
-			if (cb > 2 * unit_target) {
-				// one b over several units, by slices
-				uint32_t s0 = 0;
-				double acc = 0;
-				for (uint32_t s = 0; s < S.nslices; s++) {
-					acc += J.count[b] * slice_cost(h_slices[S.slice0 + s]);
-					if (acc >= unit_target || s + 1 == S.nslices) {
-						units.push_back(NpUnit{(int32_t)k, b, b + 1, s0, s + 1, 0});
-						s0 = s + 1;
-						acc = 0;
+			if (cb > 2 * unit_target || S.nslices > kNpUnitSlices || J.count[b] > jmax) {
+				// one b over several units: its entries in groups of at
+				// most 64 (jsub = first << 16 | count), its slices in
+				// groups of about unit_target
+				for (uint32_t j0 = 0; j0 < J.count[b]; j0 += jmax) {
+					const uint32_t jn = std::min(jmax, J.count[b] - j0);
+					const uint32_t jsub = J.count[b] > jmax ? (j0 << 16 | jn) : 0u;
+					uint32_t s0 = 0;
+					double acc = 0;
+					for (uint32_t s = 0; s < S.nslices; s++) {
+						acc += jn * slice_cost(h_slices[S.slice0 + s]);
+						if (acc >= unit_target || s + 1 == S.nslices ||
+						    s + 1 - s0 == kNpUnitSlices) {
+							units.push_back(NpUnit{(int32_t)k, b, b + 1, s0, s + 1, jsub});
+							s0 = s + 1;
+							acc = 0;
+						}
 					}
 				}
+				if (J.count[b] > 0xFFFFu)
+					return BACCHUS_FAIL("a row-list compact index with %u entries",
+							    J.count[b]);
 				total_cost += cb;
 				b++;
 				continue;
 			}
 			uint32_t e = b;
 			double acc = 0;
-			while (e < cs && acc + J.count[e] * steps <= std::max(unit_target, cb)) {
+			uint32_t entries_in = 0;
+			while (e < cs && acc + J.count[e] * steps <= std::max(unit_target, cb) &&
+			       e - b < kNpUnitSlices - 1 && entries_in + J.count[e] <= jmax) {
 				if (J.count[e] * steps > 2 * unit_target)
 					break;
 				acc += J.count[e] * steps;
+				entries_in += J.count[e];
 				e++;
 			}
 			units.push_back(NpUnit{(int32_t)k, b, e, 0, S.nslices, 0});
@@ -630,11 +762,15 @@ int np_build(bacchus_hip_operator *op, const NpContext &c)
 		t.u_end = (int32_t)h_units.size();
 		tiles.push_back(t);
 	}
-	if (max_words * 8 > 160 * 1024)
+	// LDS: the largest output block (FP64 sums), then the waves' x staging
+	const size_t xsz = prec == BACCHUS_HIP_FP64 ? 8 : 4;
+	const size_t lds = (size_t)max_words * 8 + (size_t)(kNpThreads / 64) * kNpStage * xsz;
+	if (lds > 160 * 1024)
 		return BACCHUS_FAIL("an output block of %lld states does not fit the LDS",
 				    (long long)max_words);
 	op->np_num_tiles = (int64_t)tiles.size();
-	op->np_lds_bytes = (size_t)max_words * 8;
+	op->np_lds_bytes = lds;
+	op->np_block_words = (int32_t)max_words;
 	if (getenv("BACCHUS_HIP_VERBOSE"))
 		fprintf(stderr, "np: %zu instructions, %zu units, %zu tiles, %lld elements, "
 			"%zu positions, LDS %zu B\n", virts.size(), h_units.size(), tiles.size(),
@@ -707,24 +843,29 @@ int np_launch(const bacchus_hip_operator *op, const void *x, const void *xT, dou
 {
 	if (!op->np_num_tiles)
 		return 0;
-	const dim3 grid((unsigned)op->np_num_tiles), block(kNpThreads);
-	if (op->precision == BACCHUS_HIP_FP64)
-		hipLaunchKernelGGL(np_kernel<double>, grid, block, op->np_lds_bytes, s, op->d_np_tiles,
-				   op->d_np_units, op->d_np_instr, op->d_np_slices, op->d_np_in,
-				   op->d_np_row, op->d_np_j, op->d_np_bptr, (const double *)op->d_np_w,
-				   (const double *)x, (const double *)xT, y, op->d_yT, s1, s2, s3);
-	else if (op->precision == BACCHUS_HIP_FP32)
-		hipLaunchKernelGGL(np_kernel<float>, grid, block, op->np_lds_bytes, s, op->d_np_tiles,
-				   op->d_np_units, op->d_np_instr, op->d_np_slices, op->d_np_in,
-				   op->d_np_row, op->d_np_j, op->d_np_bptr, (const float *)op->d_np_w,
-				   (const float *)x, (const float *)xT, y, op->d_yT, s1, s2, s3);
-	else
-		hipLaunchKernelGGL(np_kernel<uint16_t>, grid, block, op->np_lds_bytes, s,
-				   op->d_np_tiles, op->d_np_units, op->d_np_instr, op->d_np_slices,
-				   op->d_np_in, op->d_np_row, op->d_np_j, op->d_np_bptr,
-				   (const uint16_t *)op->d_np_w, (const float *)x, (const float *)xT, y,
-				   op->d_yT, s1, s2, s3);
-	BACCHUS_HIP_CHECK(hipGetLastError());
+	{
+		const dim3 grid((unsigned)op->np_num_tiles), block(kNpThreads);
+		const NpTile *tiles = op->d_np_tiles;
+		const size_t lds = op->np_lds_bytes;
+		const int32_t bw = op->np_block_words;
+		if (op->precision == BACCHUS_HIP_FP64)
+			hipLaunchKernelGGL(np_kernel<double>, grid, block, lds, s, tiles, op->d_np_units,
+					   op->d_np_instr, op->d_np_slices, op->d_np_in, op->d_np_row,
+					   op->d_np_j, op->d_np_bptr, (const double *)op->d_np_w,
+					   (const double *)x, (const double *)xT, y, op->d_yT, bw, s1, s2, s3);
+		else if (op->precision == BACCHUS_HIP_FP32)
+			hipLaunchKernelGGL(np_kernel<float>, grid, block, lds, s, tiles, op->d_np_units,
+					   op->d_np_instr, op->d_np_slices, op->d_np_in, op->d_np_row,
+					   op->d_np_j, op->d_np_bptr, (const float *)op->d_np_w,
+					   (const float *)x, (const float *)xT, y, op->d_yT, bw, s1, s2, s3);
+		else
+			hipLaunchKernelGGL(np_kernel<uint16_t>, grid, block, lds, s, tiles,
+					   op->d_np_units, op->d_np_instr, op->d_np_slices, op->d_np_in,
+					   op->d_np_row, op->d_np_j, op->d_np_bptr,
+					   (const uint16_t *)op->d_np_w, (const float *)x, (const float *)xT,
+					   y, op->d_yT, bw, s1, s2, s3);
+		BACCHUS_HIP_CHECK(hipGetLastError());
+	}
 	return 0;
 }
 
