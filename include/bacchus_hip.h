@@ -349,6 +349,15 @@ int bacchus_hip_symmetric_eigen(int64_t n, const double *A, double *w, double *V
 int bacchus_hip_basis_gram(const float *U, int64_t n, int64_t D, double *G);
 int bacchus_hip_basis_project(const float *U, int64_t n, int64_t D,
 			      const double *V, int64_t B, double *out);
+/* The same projection with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * partial sums live in a static device array of 2^18 doubles per GPU,
+ * held until the process ends, so n * B <= 2^18 / 65 = 4032 (larger sizes
+ * are rejected); every call on a GPU uses that GPU's array, so calls that
+ * overlap on one GPU race.  Results equal bacchus_hip_basis_project's. */
+int bacchus_hip_basis_project_static(const float *U, int64_t n, int64_t D,
+				     const double *V, int64_t B, double *out);
 int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D,
 			      const float *c, double *out, void *stream);
 /*   transform: out[j] = sum_i C[j][i] U_i for j < r (C host FP64 r x n;

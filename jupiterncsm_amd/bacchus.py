@@ -52,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
     "bacchus_hip_write_evaluation_order", "bacchus_hip_block_lanczos",
     "bacchus_hip_symmetric_eigen", "bacchus_hip_operator_load_blocks",
+    "bacchus_hip_basis_project_static",
 )
 
 
@@ -176,6 +177,8 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_basis_gram.argtypes = [vp, i64, i64, vp]
     lib.bacchus_hip_basis_project.restype = c_int
     lib.bacchus_hip_basis_project.argtypes = [vp, i64, i64, vp, i64, vp]
+    lib.bacchus_hip_basis_project_static.restype = c_int
+    lib.bacchus_hip_basis_project_static.argtypes = [vp, i64, i64, vp, i64, vp]
     lib.bacchus_hip_basis_combine.restype = c_int
     lib.bacchus_hip_basis_combine.argtypes = [vp, i64, i64, vp, vp, vp]
     lib.bacchus_hip_write_evaluation_order.restype = c_int

@@ -4,12 +4,14 @@
 // a start vector on the device.  FP32 storage, FP64 accumulation.
 #include "bacchus_internal.h"
 
+#include <mutex>
 #include <vector>
 
 namespace bacchus {
 namespace {
 
 constexpr int kChunks = 64;
+constexpr int kMaxDevices = 64;
 
 __device__ double block_reduce(double v, double *sh)
 {
@@ -70,6 +72,39 @@ __global__ __launch_bounds__(256) void project_partial(const float *__restrict__
 			t += sh[threadIdx.x + q * B];
 		partial[((int64_t)i * gridDim.x + blockIdx.x) * B + threadIdx.x] = t;
 	}
+}
+
+// out[i * B + b] = sum_c partial[(i * chunks + c) * B + b], the chunks in
+// order (one thread per output)
+__global__ void reduce_project(const double *__restrict__ partial, int n, int B, int chunks,
+			       double *__restrict__ out)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * B)
+		return;
+	const int i = t / B, b = t - i * B;
+	double s = 0;
+	for (int c = 0; c < chunks; c++)
+		s += partial[((int64_t)i * chunks + c) * B + b];
+	out[t] = s;
+}
+
+// Scratch of bacchus_hip_basis_project_static: the partial sums and the
+// reduced projections, kProjectScratch doubles in every GPU's memory (held
+// until the process ends).  Host code uses the address hipGetSymbolAddress
+// returns for the current device, never this name.
+constexpr int64_t kProjectScratch = 1 << 18;
+static __device__ __align__(256) double scratch[kProjectScratch];
+
+// projections of n basis vectors onto B device vectors into partial
+// (n * kChunks * B doubles) and their sums into finals (n * B doubles)
+void project_into(const float *U, int64_t n, int64_t D, const double *V, int64_t B,
+		  double *partial, double *finals)
+{
+	hipLaunchKernelGGL(project_partial, dim3(kChunks, (unsigned)n), dim3(256), 0, nullptr, U, D, V,
+			   (int)B, partial);
+	hipLaunchKernelGGL(reduce_project, dim3((unsigned)((n * B + 255) / 256)), dim3(256), 0,
+			   nullptr, partial, (int)n, (int)B, kChunks, finals);
 }
 
 // out[r] = sum_i c_i U_i[r] (FP64 accumulation)
@@ -155,21 +190,50 @@ int bacchus_hip_basis_project(const float *U, int64_t n, int64_t D, const double
 				    (long long)D, (long long)B);
 	double *dp = nullptr;
 	const size_t np = (size_t)n * kChunks * B;
-	BACCHUS_HIP_CHECK(hipMalloc(&dp, np * sizeof(double)));
-	hipLaunchKernelGGL(project_partial, dim3(kChunks, (unsigned)n), dim3(256), 0, nullptr, U, D, V,
-			   (int)B, dp);
-	std::vector<double> h(np);
-	const hipError_t e = hipMemcpy(h.data(), dp, np * sizeof(double), hipMemcpyDeviceToHost);
+	BACCHUS_HIP_CHECK(hipMalloc(&dp, (np + (size_t)n * B) * sizeof(double)));
+	project_into(U, n, D, V, B, dp, dp + np);
+	const hipError_t e = hipMemcpy(out, dp + np, (size_t)n * B * sizeof(double),
+				       hipMemcpyDeviceToHost);
 	(void)hipFree(dp);
 	if (e != hipSuccess)
 		return BACCHUS_FAIL("project: %s", hipGetErrorString(e));
-	for (int64_t i = 0; i < n; i++)
-		for (int64_t b = 0; b < B; b++) {
-			double s = 0;
-			for (int c = 0; c < kChunks; c++)
-				s += h[(i * kChunks + c) * B + b];
-			out[i * B + b] = s;
+	return 0;
+}
+
+int bacchus_hip_basis_project_static(const float *U, int64_t n, int64_t D, const double *V,
+				     int64_t B, double *out)
+{
+	if (!U || !V || !out || n <= 0 || D <= 0 || B <= 0 || B > 256)
+		return BACCHUS_FAIL("bad arguments (n = %lld, D = %lld, B = %lld)", (long long)n,
+				    (long long)D, (long long)B);
+	const int64_t need = n * B * (kChunks + 1);
+	if (need > kProjectScratch)
+		return BACCHUS_FAIL("n B = %lld needs %lld scratch doubles, more than the %lld of "
+				    "the static scratch (n B <= %lld)", (long long)(n * B),
+				    (long long)need, (long long)kProjectScratch,
+				    (long long)(kProjectScratch / (kChunks + 1)));
+	// the scratch array's address on the current device, looked up once per
+	// device (each GPU holds its own copy)
+	static std::mutex lock;
+	static double *address[kMaxDevices] = {};
+	int dev = 0;
+	BACCHUS_HIP_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		return BACCHUS_FAIL("device %d beyond the %d this entry point serves", dev, kMaxDevices);
+	double *base = nullptr;
+	{
+		std::lock_guard<std::mutex> g(lock);
+		if (!address[dev]) {
+			void *p = nullptr;
+			BACCHUS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+			address[dev] = static_cast<double *>(p);
 		}
+		base = address[dev];
+	}
+	const int64_t np = n * kChunks * B;
+	project_into(U, n, D, V, B, base, base + np);
+	BACCHUS_HIP_CHECK(hipMemcpy(out, base + np, (size_t)(n * B) * sizeof(double),
+				    hipMemcpyDeviceToHost));
 	return 0;
 }
 
