@@ -66,6 +66,8 @@ struct TileWork {
 	int32_t t_begin;   // tasks [t_begin, t_end)
 	int32_t t_end;
 	int32_t space;     // 0 normal, 1 transposed
+	int32_t group;     // input group of its instructions (0 without
+			   // input groups)
 };
 
 // One unit of a tile's work: row-side entries [jb, je) (rows of the tile,
@@ -101,12 +103,23 @@ struct PlanParams {
 	int xcds = 0;               // > 1: group output blocks per XCD
 	bool skip_np = false;       // leave NP instructions out (np_kernel
 				    // applies them)
+	// > 1: input groups.  Every basis block is given one of in_groups
+	// groups (balanced by the work that reads it, added to *group_load
+	// when that is set, so several operators share one balance), every
+	// tile holds instructions of one group only, and with xcds > 1 the
+	// tiles of group g go to XCD g: an XCD's L2 then holds the input
+	// vectors of its own blocks only.  Output rows reached from several
+	// groups are flushed atomically.
+	int in_groups = 0;
+	std::vector<double> *group_load = nullptr;
 };
 struct Plan {
 	std::vector<TileWork> tiles;
 	std::vector<Task> tasks;
 	std::vector<double> cost;   // multiplications x bt of every tile
 	std::vector<int> bt;        // vectors of every tile
+	// virtual instructions of tile t: vi_order[vi_begin, vi_end)
+	std::vector<int32_t> vi_order;
 	int64_t max_tile_elems = 1;
 };
 

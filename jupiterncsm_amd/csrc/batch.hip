@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -126,7 +127,10 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int lq = lbt - 2;       // log2(lanes per slot)
 	const int nrow = w.po_end - w.po_begin;
 	const int nn = w.nn_out;
-	const int twords = (nrow * nn) << lbt;
+	// an output element's bt sums at a pitch of bt + 1 doubles, so the
+	// lanes' LDS adds to different outputs fall into different banks
+	const int pitch = (1 << lbt) + 1;
+	const int twords = nrow * nn * pitch;
 	float *ys = w.space ? o.yT : o.y;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x)
 		tile[t] = 0.0;
@@ -153,7 +157,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 		if (I.type == 3) {
 			// every lane holds kLaneEntries lane-side entries (slot
 			// chunks), so a broadcast row-side entry puts 2 kLaneEntries
-			// independent loads in flight
+			// independent loads in flight; the loads of row-side entry
+			// t + 1 are issued before the products of entry t, so two
+			// entries' loads overlap (the element rows stream from HBM)
 			const VT *vblk = V + I.v_off;
 			const uint32_t dn = (uint32_t)I.dn;
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += kLaneEntries * G) {
@@ -179,62 +185,78 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 				}
 				auto rows_pass = [&](auto full) {
-				int cur = -1;
-				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
-					const uint32_t cnt = min(64u, T.je - j0);
-					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
-					const uint2 pl = o.ent[jl];
-					const uint32_t prow = o.keys[jl];
-					for (uint32_t t = 0; t < cnt; t++) {
-						const int row = (int)__builtin_amdgcn_readlane(prow, t);
-						if (row != cur) {
-							if (cur >= 0) {
-								const int r = cur - w.po_begin;
+				constexpr bool kFull = decltype(full)::value;
+				float va[2][kLaneEntries];
+				float4 xa[2][kLaneEntries];
+				uint2 pl;
+				// loads of row-side entry t (lane t of pl) into buffer buf
+				auto issue = [&](uint32_t t, auto buf) {
+					constexpr int k = decltype(buf)::value;
+					const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
+					const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+					const size_t vrow = (size_t)dn * (py & kMask);
+					const size_t xrow = (size_t)nn_in * px;
 #pragma unroll
-								for (int c = 0; c < kLaneEntries; c++)
-									if (vl[c]) {
-										const float4 s4 = make_float4(sgn[c] * sc.x, sgn[c] * sc.y,
-													      sgn[c] * sc.z, sgn[c] * sc.w);
-										add4_lds(&tile[((r * nn + (int)outc[c]) << lbt) + q4], s4,
-											 acc[c]);
-									}
-							}
-#pragma unroll
-							for (int c = 0; c < kLaneEntries; c++)
-								acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-							cur = row;
+					for (int c = 0; c < kLaneEntries; c++) {
+						if (kFull || c < nc) {
+							va[k][c] = Load<VT>::at(vblk, vrow + col[c]);
+							xa[k][c] = ld4(xb + (xrow + xin[c]) * B);
+						} else {
+							va[k][c] = 0.f;
+							xa[k][c] = make_float4(0.f, 0.f, 0.f, 0.f);
 						}
-						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
-						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-						const size_t vrow = (size_t)dn * (py & kMask);
-						const size_t xrow = (size_t)nn_in * px;
-						float va[kLaneEntries];
-						float4 xa[kLaneEntries];
-#pragma unroll
-						for (int c = 0; c < kLaneEntries; c++) {
-							if (decltype(full)::value || c < nc) {
-								va[c] = Load<VT>::at(vblk, vrow + col[c]);
-								xa[c] = ld4(xb + (xrow + xin[c]) * B);
-							} else {
-								va[c] = 0.f;
-								xa[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-							}
-						}
-#pragma unroll
-						for (int c = 0; c < kLaneEntries; c++)
-							fma4(acc[c], sign_of(va[c], py), xa[c]);
 					}
-				}
-				if (cur >= 0) {
+				};
+				int cur = -1;
+				auto flush = [&]() {
 					const int r = cur - w.po_begin;
 #pragma unroll
 					for (int c = 0; c < kLaneEntries; c++)
 						if (vl[c]) {
 							const float4 s4 = make_float4(sgn[c] * sc.x, sgn[c] * sc.y,
 										      sgn[c] * sc.z, sgn[c] * sc.w);
-							add4_lds(&tile[((r * nn + (int)outc[c]) << lbt) + q4], s4, acc[c]);
+							add4_lds(&tile[(r * nn + (int)outc[c]) * pitch + q4], s4, acc[c]);
 						}
+				};
+				// products of entry t from buffer buf, after a flush when
+				// its row differs from the current one
+				auto consume = [&](uint32_t t, uint32_t prow, auto buf) {
+					constexpr int k = decltype(buf)::value;
+					const int row = (int)__builtin_amdgcn_readlane(prow, t);
+					if (row != cur) {
+						if (cur >= 0)
+							flush();
+#pragma unroll
+						for (int c = 0; c < kLaneEntries; c++)
+							acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+						cur = row;
+					}
+					const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+#pragma unroll
+					for (int c = 0; c < kLaneEntries; c++)
+						fma4(acc[c], sign_of(va[k][c], py), xa[k][c]);
+				};
+				using B0 = std::integral_constant<int, 0>;
+				using B1 = std::integral_constant<int, 1>;
+				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
+					const uint32_t cnt = min(64u, T.je - j0);
+					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
+					pl = o.ent[jl];
+					const uint32_t prow = o.keys[jl];
+					issue(0u, B0{});
+					for (uint32_t t = 0; t < cnt; t += 2) {
+						if (t + 1 < cnt)
+							issue(t + 1, B1{});
+						consume(t, prow, B0{});
+						if (t + 1 < cnt) {
+							if (t + 2 < cnt)
+								issue(t + 2, B0{});
+							consume(t + 1, prow, B1{});
+						}
+					}
 				}
+				if (cur >= 0)
+					flush();
 				};
 				if (nc == kLaneEntries)
 					rows_pass(std::true_type{});
@@ -278,7 +300,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						break;
 					if (rw[u] != cur) {
 						if (cur >= 0 && valid)
-							add4_lds(&tile[(((cur - w.po_begin) * nn + (int)nl) << lbt) + q4], sc, a);
+							add4_lds(&tile[((cur - w.po_begin) * nn + (int)nl) * pitch + q4], sc, a);
 						a = make_float4(0.f, 0.f, 0.f, 0.f);
 						cur = rw[u];
 					}
@@ -286,7 +308,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				}
 			}
 			if (cur >= 0 && valid)
-				add4_lds(&tile[(((cur - w.po_begin) * nn + (int)nl) << lbt) + q4], sc, a);
+				add4_lds(&tile[((cur - w.po_begin) * nn + (int)nl) * pitch + q4], sc, a);
 		} else if (I.type == 2) {
 			for (int n0 = (int)T.e0; n0 < (int)T.e1; n0 += G) {
 				const int n = n0 + slot;
@@ -303,7 +325,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 						const int row = (int)__builtin_amdgcn_readlane(prow, t);
 						if (row != cur) {
 							if (cur >= 0 && valid)
-								add4_lds(&tile[(((cur - w.po_begin) * nn + n) << lbt) + q4], sc, a0);
+								add4_lds(&tile[((cur - w.po_begin) * nn + n) * pitch + q4], sc, a0);
 							a0 = make_float4(0.f, 0.f, 0.f, 0.f);
 							cur = row;
 						}
@@ -314,15 +336,15 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					}
 				}
 				if (cur >= 0 && valid)
-					add4_lds(&tile[(((cur - w.po_begin) * nn + n) << lbt) + q4], sc, a0);
+					add4_lds(&tile[((cur - w.po_begin) * nn + n) * pitch + q4], sc, a0);
 			}
 		}
 	}
 	__syncthreads();
 	float *yblk = ys + ((size_t)w.y_off + (size_t)nn * w.po_begin) * B + b0;
 	for (int t = threadIdx.x; t < twords; t += blockDim.x) {
-		const int e = t >> lbt, c = t & (bt - 1);
-		if (b0 + c >= B)
+		const int e = t / pitch, c = t - e * pitch;
+		if (c >= bt || b0 + c >= B)
 			continue;
 		const float v = (float)tile[t];
 		float *dst = yblk + (size_t)e * B + c;
@@ -334,31 +356,38 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	}
 }
 
-// Persistent grid over a work queue: each workgroup takes the next item
-// (heaviest first) from *next until the list is exhausted.  *next is
-// cleared by reset_queue before every launch.
+// Persistent grid over nq work queues (items qbegin[q]..qbegin[q + 1] of
+// queue q, heaviest first): a workgroup takes items from the queue of its
+// XCD (workgroup i runs on XCD i mod kXcds) until it is exhausted, then
+// from the others in turn.  The queue heads next[q] are cleared by
+// reset_queue before every launch.
 __global__ __launch_bounds__(kTileThreads) void batch_kernel(
-	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items, int nitems,
-	int *__restrict__ next)
+	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
+	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
 {
 	extern __shared__ double tile[];
 	__shared__ int s_item;
-	for (;;) {
-		if (threadIdx.x == 0)
-			s_item = atomicAdd(next, 1);
-		__syncthreads();
-		const int it = __builtin_amdgcn_readfirstlane(s_item);
-		__syncthreads();
-		if (it >= nitems)
-			break;
-		const BatchItem bi = items[it];
-		const BatchOp o = ops[bi.op];
-		const TileWork w = o.tiles[bi.tile];
-		if (o.precision == BACCHUS_HIP_BF16)
-			batch_tile<uint16_t>(o, w, bi.b0, bi.lbt, tile);
-		else
-			batch_tile<float>(o, w, bi.b0, bi.lbt, tile);
-		__syncthreads();
+	const int home = (int)(blockIdx.x % kXcds) % nq;
+	for (int qi = 0; qi < nq; qi++) {
+		const int q = (home + qi) % nq;
+		const int base = qbegin[q], count = qbegin[q + 1] - base;
+		for (;;) {
+			if (threadIdx.x == 0)
+				s_item = atomicAdd(next + q, 1);
+			__syncthreads();
+			const int it = __builtin_amdgcn_readfirstlane(s_item);
+			__syncthreads();
+			if (it >= count)
+				break;
+			const BatchItem bi = items[base + it];
+			const BatchOp o = ops[bi.op];
+			const TileWork w = o.tiles[bi.tile];
+			if (o.precision == BACCHUS_HIP_BF16)
+				batch_tile<uint16_t>(o, w, bi.b0, bi.lbt, tile);
+			else
+				batch_tile<float>(o, w, bi.b0, bi.lbt, tile);
+			__syncthreads();
+		}
 	}
 }
 
@@ -384,10 +413,10 @@ __global__ void unpad_cols(const float *__restrict__ src, int64_t D, int B, int 
 	dst[i] = src[r * Bp + (i - r * B)];
 }
 
-__global__ void reset_queue(int *next)
+__global__ void reset_queue(int *next, int nq)
 {
-	if (threadIdx.x == 0 && blockIdx.x == 0)
-		*next = 0;
+	if (blockIdx.x == 0 && (int)threadIdx.x < nq)
+		next[threadIdx.x] = 0;
 }
 
 __global__ void fill_float(float *p, int64_t n, float v)
@@ -423,7 +452,9 @@ struct bacchus_hip_batch {
 	std::vector<BatchOp> h_ops;
 	BatchOp *d_ops = nullptr;
 	BatchItem *d_items = nullptr;
-	int *d_next = nullptr;  // work-queue head
+	int *d_next = nullptr;    // work-queue heads (kXcds)
+	int *d_qbegin = nullptr;  // first item of every queue (nq + 1)
+	int nq = 1;
 	int64_t num_items = 0;
 	size_t lds_bytes = 0;
 	int grid = 0;
@@ -441,9 +472,11 @@ struct bacchus_hip_batch {
 		for (auto *p : d_tasks)
 			(void)hipFree(p);
 		(void)hipFree(d_items);
+		(void)hipFree(d_qbegin);
 		d_tiles.clear();
 		d_tasks.clear();
 		d_items = nullptr;
+		d_qbegin = nullptr;
 	}
 	~bacchus_hip_batch()
 	{
@@ -497,7 +530,12 @@ int build_plan(bacchus_hip_batch_t *bt)
 		double cost;
 		BatchItem item;
 	};
-	std::vector<Cand> all;
+	// input groups, one work queue per XCD (balanced over all operators)
+	int groups = kXcds;
+	if (const char *ge = getenv("BACCHUS_HIP_BATCH_GROUPS"))
+		groups = std::max(1, std::min(kXcds, atoi(ge)));
+	std::vector<double> group_load(groups, 0.0);
+	std::vector<std::vector<Cand>> queues(groups);
 	int64_t max_words = 1;
 	for (int i = 0; i < nops; i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
@@ -506,12 +544,16 @@ int build_plan(bacchus_hip_batch_t *bt)
 		const int vec = std::max(4, std::min(max_vec, pow2_floor(Bp) * (pow2_floor(Bp) < Bp ? 2 : 1)));
 		PlanParams pp;
 		pp.target = target;
+		pp.in_groups = groups;
+		pp.group_load = &group_load;
 		// row widths: neutron states (normal space), proton states
 		// (transposed space)
 		int64_t widest = 1;
 		for (int64_t b = 0; b < nb; b++)
 			widest = std::max({widest, op->block_nn[b], op->block_np[b]});
-		pp.tile_elems = std::max<int64_t>(words, 4 * widest);
+		// the LDS tile holds rows x nn x (bt + 1) doubles (the pitch of
+		// batch_tile): the plan's rows x nn x bt at most 4/5 of the words
+		pp.tile_elems = std::max<int64_t>(words * 4 / 5, 4 * widest);
 		pp.bt.resize(2 * nb);
 		for (int64_t slot = 0; slot < 2 * nb; slot++) {
 			// widest vector chunk that still leaves kMinTileRows rows in
@@ -542,31 +584,49 @@ int build_plan(bacchus_hip_batch_t *bt)
 						    hipMemcpyHostToDevice));
 		bt->h_ops[i].tiles = dt;
 		bt->h_ops[i].tasks = dk;
-		max_words = std::max(max_words, plan.max_tile_elems);
+		for (size_t t = 0; t < plan.tiles.size(); t++) {
+			const TileWork &tw = plan.tiles[t];
+			max_words = std::max<int64_t>(max_words, int64_t(tw.po_end - tw.po_begin) *
+									 tw.nn_out * (plan.bt[t] + 1));
+		}
 		for (size_t t = 0; t < plan.tiles.size(); t++) {
 			const int v = plan.bt[t];
 			int lbt = 0;
 			while ((1 << lbt) < v)
 				lbt++;
 			for (int b0 = 0; b0 < Bp; b0 += v)
-				all.push_back(Cand{plan.cost[t], BatchItem{i, (int32_t)t, b0, lbt}});
+				queues[plan.tiles[t].group].push_back(
+					Cand{plan.cost[t], BatchItem{i, (int32_t)t, b0, lbt}});
 		}
 	}
-	std::stable_sort(all.begin(), all.end(),
-			 [](const Cand &a, const Cand &b) { return a.cost > b.cost; });
-	std::vector<BatchItem> items(all.size());
-	for (size_t k = 0; k < all.size(); k++)
-		items[k] = all[k].item;
-	if (items.size() > 0x7FFFFFFFull)
-		return BACCHUS_FAIL("more than 2^31 batch work items");
+	std::vector<BatchItem> items;
+	std::vector<int> qbegin(1, 0);
+	for (auto &q : queues) {
+		std::stable_sort(q.begin(), q.end(),
+				 [](const Cand &a, const Cand &b) { return a.cost > b.cost; });
+		for (const Cand &c : q)
+			items.push_back(c.item);
+		if (items.size() > 0x7FFFFFFFull)
+			return BACCHUS_FAIL("more than 2^31 batch work items");
+		qbegin.push_back((int)items.size());
+	}
 	bt->num_items = (int64_t)items.size();
+	bt->nq = groups;
 	BACCHUS_HIP_CHECK(hipMalloc(&bt->d_items, std::max<size_t>(16, items.size() * sizeof(BatchItem))));
 	if (!items.empty())
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(BatchItem),
 					    hipMemcpyHostToDevice));
+	BACCHUS_HIP_CHECK(hipMalloc(&bt->d_qbegin, qbegin.size() * sizeof(int)));
+	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_qbegin, qbegin.data(), qbegin.size() * sizeof(int),
+				    hipMemcpyHostToDevice));
+	if (max_words > kBatchTileWords)
+		return BACCHUS_FAIL("a batched tile needs %lld doubles of LDS (%lld at most)",
+				    (long long)max_words, (long long)kBatchTileWords);
 	bt->lds_bytes = (size_t)max_words * sizeof(double);
 	const int wg_per_cu = tu.workgroups_per_cu > 0 ? tu.workgroups_per_cu : 8;
-	bt->grid = (int)std::max<int64_t>(1, std::min<int64_t>(bt->num_items, (int64_t)num_cu * wg_per_cu));
+	// a multiple of kXcds, so every XCD has the same number of workgroups
+	const int64_t want = std::min<int64_t>(bt->num_items, (int64_t)num_cu * wg_per_cu);
+	bt->grid = (int)std::max<int64_t>(kXcds, (want + kXcds - 1) / kXcds * kXcds);
 	return upload_ops(bt);
 }
 
@@ -585,9 +645,9 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 		}
 	}
 	if (bt->num_items) {
-		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next);
+		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next, bt->nq);
 		hipLaunchKernelGGL(batch_kernel, dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
-				   bt->d_ops, bt->d_items, (int)bt->num_items, bt->d_next);
+				   bt->d_ops, bt->d_items, bt->d_qbegin, bt->nq, bt->d_next);
 	}
 	for (size_t i = 0; i < bt->ops.size(); i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
@@ -677,11 +737,11 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 			return BACCHUS_FAIL("operator %d: D x B = %lld exceeds 2^31", i,
 					    (long long)(ops[i]->dimension * num_vectors[i]));
 		for (size_t b = 0; b < ops[i]->block_nn.size(); b++)
-			if (std::max(ops[i]->block_nn[b], ops[i]->block_np[b]) > kBatchTileWords)
+			if (std::max(ops[i]->block_nn[b], ops[i]->block_np[b]) > kBatchTileWords / 5)
 				return BACCHUS_FAIL("operator %d has a block of %lld x %lld states; the "
 						    "batched tile holds rows of at most %lld", i,
 						    (long long)ops[i]->block_np[b], (long long)ops[i]->block_nn[b],
-						    (long long)kBatchTileWords);
+						    (long long)(kBatchTileWords / 5));
 	}
 	auto *bt = new bacchus_hip_batch;
 	if (tuning)
@@ -738,7 +798,7 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 		bt->flops += 2.0 * op->mults * B;
 	}
 	if (hipMalloc(&bt->d_ops, num_ops * sizeof(BatchOp)) != hipSuccess ||
-	    hipMalloc(&bt->d_next, sizeof(int)) != hipSuccess)
+	    hipMalloc(&bt->d_next, kXcds * sizeof(int)) != hipSuccess)
 		return fail(BACCHUS_FAIL("hipMalloc of the descriptors failed"));
 	if (int rc = build_plan(bt))
 		return fail(rc);

@@ -570,91 +570,158 @@ namespace bacchus {
 // do not form the tail; every tile gets tasks: N instructions in entry
 // ranges over all rows, P and NP instructions as pieces of the contiguous
 // proton segment of the tile's rows times ranges of neutron entries (NP)
-// or neutron states (P).
+// or neutron states (P).  With input groups the instructions of a block
+// are split by the group of their input block first, and every group gets
+// its own tiles.
 int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 {
 	const std::vector<uint32_t> &rowptr = op->h_rowptr;
 	const std::vector<TInstr> &tinstr = op->h_tinstr;
 	const int64_t nb = (int64_t)op->block_np.size();
 	out = Plan();
+	out.vi_order.resize(tinstr.size());
+	std::iota(out.vi_order.begin(), out.vi_order.end(), 0);
+	auto slot_end = [&](int64_t slot) {
+		return pp.skip_np ? op->blk_np[slot] : op->blk_vi[slot + 1];
+	};
+	auto dims = [&](int64_t slot, int64_t &nn, int64_t &np) {
+		const int64_t b = slot % nb;
+		nn = slot >= nb ? op->block_np[b] : op->block_nn[b];
+		np = slot >= nb ? op->block_nn[b] : op->block_np[b];
+	};
+	// multiplications of virtual instruction v over rows [po0, po1)
+	auto icost = [&](const TInstr &t, int64_t nn, int64_t po0, int64_t po1) {
+		const double e = double(rowptr[t.p_rows + po1] - rowptr[t.p_rows + po0]);
+		const double w = t.type == 3 ? double(t.n_len) : double(nn);
+		return w * e + w * 0.25 * double(po1 - po0);
+	};
+	// input group of every virtual instruction
+	std::vector<int32_t> vgroup(tinstr.size(), 0);
+	const int G = std::max(1, pp.in_groups);
+	if (G > 1) {
+		std::map<int64_t, int64_t> block_at;
+		for (int64_t b = 0; b < nb; b++)
+			block_at[op->block_off[b]] = b;
+		std::vector<int64_t> vblock(tinstr.size(), 0);
+		std::vector<double> bcost(nb, 0.0);
+		for (int64_t slot = 0; slot < 2 * nb; slot++) {
+			int64_t nn, np;
+			dims(slot, nn, np);
+			const int bt = pp.bt.empty() ? 1 : pp.bt[slot];
+			for (int32_t v = op->blk_vi[slot]; v < slot_end(slot); v++) {
+				const auto it = block_at.find(tinstr[v].x_off);
+				if (it == block_at.end())
+					return BACCHUS_FAIL("instruction input at %lld is no block start",
+							    (long long)tinstr[v].x_off);
+				vblock[v] = it->second;
+				bcost[it->second] += icost(tinstr[v], nn, 0, np) * bt;
+			}
+		}
+		std::vector<double> own(G, 0.0);
+		std::vector<double> &load = pp.group_load ? *pp.group_load : own;
+		load.resize(G, 0.0);
+		std::vector<int64_t> bo(nb);
+		std::iota(bo.begin(), bo.end(), 0);
+		std::stable_sort(bo.begin(), bo.end(),
+				 [&](int64_t a, int64_t b) { return bcost[a] > bcost[b]; });
+		std::vector<int32_t> bgroup(nb, 0);
+		for (int64_t b : bo) {
+			if (bcost[b] == 0)
+				continue;
+			const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+			bgroup[b] = g;
+			load[g] += bcost[b];
+		}
+		for (size_t v = 0; v < tinstr.size(); v++)
+			vgroup[v] = bgroup[vblock[v]];
+	}
 	std::vector<int> tile_bt;
 	for (int64_t slot = 0; slot < 2 * nb; slot++) {
 		const int32_t first = op->blk_vi[slot];
-		int32_t last = pp.skip_np ? op->blk_np[slot] : op->blk_vi[slot + 1];
+		const int32_t last = slot_end(slot);
 		if (first == last)
 			continue;
 		const int64_t b = slot % nb;
 		const int space = slot >= nb;
 		// rows x columns of the block in its space
-		const int64_t nn = space ? op->block_np[b] : op->block_nn[b];
-		const int64_t np = space ? op->block_nn[b] : op->block_np[b];
+		int64_t nn, np;
+		dims(slot, nn, np);
 		const int bt = pp.bt.empty() ? 1 : pp.bt[slot];
 		if (nn * bt > pp.tile_elems)
 			return BACCHUS_FAIL("basis block %lld has rows of %lld states; a tile "
 					    "of %d vectors holds at most %lld", (long long)b,
 					    (long long)nn, bt, (long long)(pp.tile_elems / bt));
-		// cost of every (instruction, row) and of every row
-		const size_t nv = (size_t)(last - first);
-		std::vector<double> cost(nv * np, 0.0), row_cost(np, 0.0);
-		for (size_t k = 0; k < nv; k++) {
-			const TInstr &t = tinstr[first + k];
-			for (int64_t po = 0; po < np; po++) {
-				double c;
-				if (t.type == 3)
-					c = double(t.n_len) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
-					    t.n_len * 0.25;
-				else
-					c = double(nn) * (rowptr[t.p_rows + po + 1] - rowptr[t.p_rows + po]) +
-					    nn * 0.25;
-				c *= bt;
-				cost[k * np + po] = c;
-				row_cost[po] += c;
-			}
-		}
-		const int64_t rows_max = std::max<int64_t>(1, pp.tile_elems / (nn * bt));
-		int64_t po0 = 0;
-		while (po0 < np) {
-			int64_t po1 = po0 + 1;
-			double c = row_cost[po0];
-			while (po1 < np && po1 - po0 < rows_max && c + row_cost[po1] <= pp.target) {
-				c += row_cost[po1];
-				po1++;
-			}
-			out.max_tile_elems = std::max(out.max_tile_elems, (po1 - po0) * nn * bt);
-			TileWork w{};
-			w.y_off = op->block_off[b];
-			w.nn_out = (int32_t)nn;
-			w.po_begin = (int32_t)po0;
-			w.po_end = (int32_t)po1;
-			w.space = space;
-			const int pieces = c > 2.0 * pp.target ? (int)std::ceil(c / pp.target) : 1;
-			if (pieces == 1) {
-				w.vi_begin = first;
-				w.vi_end = last;
-				w.shared = 0;
-				out.tiles.push_back(w);
-				out.cost.push_back(c);
-				tile_bt.push_back(bt);
-			} else {
-				const double per = c / pieces;
-				int32_t begin = 0;
-				double acc = 0;
-				for (size_t k = 0; k < nv; k++) {
-					for (int64_t po = po0; po < po1; po++)
-						acc += cost[k * np + po];
-					if (acc >= per || k + 1 == nv) {
-						w.vi_begin = first + begin;
-						w.vi_end = first + (int32_t)k + 1;
-						w.shared = 1;
-						out.tiles.push_back(w);
-						out.cost.push_back(acc);
-						tile_bt.push_back(bt);
-						begin = (int32_t)k + 1;
-						acc = 0;
-					}
+		// the slot's instructions by group (stable: instruction order
+		// within a group)
+		int32_t *ord = out.vi_order.data();
+		std::stable_sort(ord + first, ord + last,
+				 [&](int32_t u, int32_t w) { return vgroup[u] < vgroup[w]; });
+		int ngroups = 0;
+		for (int32_t k = first; k < last; k++)
+			ngroups += k == first || vgroup[ord[k]] != vgroup[ord[k - 1]];
+		for (int32_t gfirst = first; gfirst < last;) {
+			const int32_t g = vgroup[ord[gfirst]];
+			int32_t glast = gfirst;
+			while (glast < last && vgroup[ord[glast]] == g)
+				glast++;
+			// cost of every (instruction, row) and of every row
+			const size_t nv = (size_t)(glast - gfirst);
+			std::vector<double> cost(nv * np, 0.0), row_cost(np, 0.0);
+			for (size_t k = 0; k < nv; k++) {
+				const TInstr &t = tinstr[ord[gfirst + k]];
+				for (int64_t po = 0; po < np; po++) {
+					const double c = icost(t, nn, po, po + 1) * bt;
+					cost[k * np + po] = c;
+					row_cost[po] += c;
 				}
 			}
-			po0 = po1;
+			const int64_t rows_max = std::max<int64_t>(1, pp.tile_elems / (nn * bt));
+			int64_t po0 = 0;
+			while (po0 < np) {
+				int64_t po1 = po0 + 1;
+				double c = row_cost[po0];
+				while (po1 < np && po1 - po0 < rows_max && c + row_cost[po1] <= pp.target) {
+					c += row_cost[po1];
+					po1++;
+				}
+				out.max_tile_elems = std::max(out.max_tile_elems, (po1 - po0) * nn * bt);
+				TileWork w{};
+				w.y_off = op->block_off[b];
+				w.nn_out = (int32_t)nn;
+				w.po_begin = (int32_t)po0;
+				w.po_end = (int32_t)po1;
+				w.space = space;
+				w.group = g;
+				const int pieces = c > 2.0 * pp.target ? (int)std::ceil(c / pp.target) : 1;
+				if (pieces == 1) {
+					w.vi_begin = gfirst;
+					w.vi_end = glast;
+					w.shared = ngroups > 1;
+					out.tiles.push_back(w);
+					out.cost.push_back(c);
+					tile_bt.push_back(bt);
+				} else {
+					const double per = c / pieces;
+					int32_t begin = 0;
+					double acc = 0;
+					for (size_t k = 0; k < nv; k++) {
+						for (int64_t po = po0; po < po1; po++)
+							acc += cost[k * np + po];
+						if (acc >= per || k + 1 == nv) {
+							w.vi_begin = gfirst + begin;
+							w.vi_end = gfirst + (int32_t)k + 1;
+							w.shared = 1;
+							out.tiles.push_back(w);
+							out.cost.push_back(acc);
+							tile_bt.push_back(bt);
+							begin = (int32_t)k + 1;
+							acc = 0;
+						}
+					}
+				}
+				po0 = po1;
+			}
+			gfirst = glast;
 		}
 	}
 	// heaviest first
@@ -679,10 +746,14 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 		// in the same instruction order) goes to one XCD and its tiles are
 		// dispatched together there, sharing that XCD's L2.  Blocks are
 		// balanced over the XCDs by cost; holes are padded with empty
-		// tiles.
+		// tiles.  With input groups, group g takes the place of the
+		// output blocks: its tiles go to one XCD.
 		std::map<std::pair<int, int64_t>, std::vector<size_t>> groups;
 		for (size_t k = 0; k < out.tiles.size(); k++)
-			groups[{out.tiles[k].space, out.tiles[k].y_off}].push_back(k);
+			if (G > 1)
+				groups[{0, out.tiles[k].group}].push_back(k);
+			else
+				groups[{out.tiles[k].space, out.tiles[k].y_off}].push_back(k);
 		std::vector<std::pair<double, std::vector<size_t>>> gl;
 		for (auto &g : groups) {
 			double c = 0;
@@ -729,7 +800,8 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 		TileWork &t = out.tiles[ti];
 		const uint64_t task_mults = std::max<uint64_t>(4096, pp.task_mults / out.bt[ti]);
 		t.t_begin = (int32_t)out.tasks.size();
-		for (int32_t v = t.vi_begin; v < t.vi_end; v++) {
+		for (int32_t k = t.vi_begin; k < t.vi_end; k++) {
+			const int32_t v = out.vi_order[k];
 			const TInstr &I = tinstr[v];
 			const uint32_t jb = rowptr[I.p_rows + t.po_begin], je = rowptr[I.p_rows + t.po_end];
 			if (je <= jb)
@@ -1398,6 +1470,84 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	}
 	op->compact_elements = (int64_t)W.size();
 
+	// --- NP virtual instructions of one output block that share their
+	// lane copy, force class and input block are merged into one: their
+	// row-side entries, by output row, in a new CSR whose entries carry the
+	// element row directly (the instruction's first element + compact
+	// index x row stride, with the entry's phase in bit 31).  A lane's sum
+	// for an output then runs over the row entries of every merged
+	// instruction before it goes to the tile, which takes that many fewer
+	// LDS adds.  Merged instructions have v_off 0 and dn 1, so the kernels
+	// read them unchanged ---
+	{
+		for (int64_t slot = 0; slot < 2 * nb; slot++) {
+			auto &vs = per_block[slot];
+			const int64_t nrows = slot < nb ? op->block_np[slot] : op->block_nn[slot - nb];
+			std::map<std::tuple<uint32_t, uint32_t, int, int64_t, int32_t, int32_t>,
+				 std::vector<size_t>> groups;
+			for (size_t k = 0; k < vs.size(); k++) {
+				const TInstr &t = vs[k];
+				if (t.type != 3)
+					continue;
+				groups[std::make_tuple(t.n_base, t.n_len, (int)t.force, t.x_off, t.nn_in,
+						       t.expanded)].push_back(k);
+			}
+			std::vector<char> drop(vs.size(), 0);
+			std::vector<TInstr> merged;
+			for (const auto &g : groups) {
+				const std::vector<size_t> &ks = g.second;
+				if (ks.size() < 2)
+					continue;
+				bool fits = true;
+				for (size_t k : ks) {
+					const TInstr &t = vs[k];
+					const uint32_t a = rowptr[t.p_rows], b = rowptr[t.p_rows + nrows];
+					uint32_t bmax = 0;
+					for (uint32_t e = a; e < b; e++)
+						bmax = std::max(bmax, entries[e].y & kIndexMask);
+					if (t.v_off + (int64_t)bmax * t.dn >= (int64_t)kIndexMask)
+						fits = false;
+				}
+				if (!fits)
+					continue;
+				const uint32_t base = (uint32_t)rowptr.size();
+				if (rowptr.size() + nrows + 1 > 0xFFFFFFFFull)
+					return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+				for (int64_t r = 0; r < nrows; r++) {
+					rowptr.push_back((uint32_t)entries.size());
+					for (size_t k : ks) {
+						const TInstr &t = vs[k];
+						for (uint32_t e = rowptr[t.p_rows + r]; e < rowptr[t.p_rows + r + 1]; e++) {
+							const uint2 en = entries[e];
+							const uint32_t row = (uint32_t)(t.v_off + (int64_t)(en.y & kIndexMask) * t.dn);
+							entries.push_back(make_uint2(en.x, row | (en.y & ~kIndexMask)));
+							keys.push_back((uint32_t)r);
+							pad_entry.push_back(0);
+						}
+					}
+				}
+				rowptr.push_back((uint32_t)entries.size());
+				if (entries.size() > 0xFFFFFFFFull)
+					return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
+				TInstr m = vs[ks[0]];
+				m.v_off = 0;
+				m.dn = 1;
+				m.p_rows = base;
+				merged.push_back(m);
+				for (size_t k : ks)
+					drop[k] = 1;
+			}
+			if (merged.empty())
+				continue;
+			std::vector<TInstr> kept;
+			for (size_t k = 0; k < vs.size(); k++)
+				if (!drop[k])
+					kept.push_back(vs[k]);
+			kept.insert(kept.end(), merged.begin(), merged.end());
+			vs.swap(kept);
+		}
+	}
+
 	// --- host copies for the planner; virtual instructions of a block
 	// sorted by element sub-block so neighbouring tasks share elements ---
 	op->blk_vi.assign(2 * nb + 1, 0);
@@ -1455,6 +1605,8 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	pp.skip_np = op->np_path;
 	if (const char *xe = getenv("BACCHUS_HIP_XCDS"))
 		pp.xcds = atoi(xe);
+	if (const char *ge = getenv("BACCHUS_HIP_IN_GROUPS"))
+		pp.in_groups = atoi(ge);
 	Plan plan;
 	if (int rc = make_plan(op, pp, plan))
 		return fail(rc);
