@@ -59,9 +59,24 @@ __device__ __forceinline__ float sign_of(float v, uint32_t m)
 	return __int_as_float(__float_as_int(v) ^ (int)(m & 0x80000000u));
 }
 
+// The operator arrays reach the kernel through BatchOp descriptors in
+// memory, so the compiler sees generic pointers and would emit flat loads
+// (which also count against the LDS counter: every wait for an LDS add or
+// a scalar load would then wait for all loads in flight).  Loads go through
+// global-address-space pointers instead.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BACCHUS_GLOBAL __attribute__((address_space(1)))
+#else
+#define BACCHUS_GLOBAL
+#endif
+template <typename T> __device__ __forceinline__ const T BACCHUS_GLOBAL *global_of(const T *p)
+{
+	return (const T BACCHUS_GLOBAL *)p;
+}
+
 __device__ __forceinline__ float4 ld4(const float *p)
 {
-	return *reinterpret_cast<const float4 *>(p);
+	return *reinterpret_cast<const float4 BACCHUS_GLOBAL *>(global_of(p));
 }
 
 __device__ __forceinline__ void fma4(float4 &a, float v, const float4 &x)
@@ -95,14 +110,39 @@ __device__ __forceinline__ void add4_lds(double *t, const float4 &s, const float
 	atomicAdd(t + 3, (double)(s.w * v.w));
 }
 
+// Task and virtual-instruction descriptors travel as dwords in the lanes
+// of one VGPR (lane i holds dword i): a wave loads the next task's ahead of
+// time with one vector load and reads the fields out with readlane when it
+// gets there, so no dependent descriptor load sits between two tasks.
+static_assert(sizeof(Task) == 8 * sizeof(uint32_t), "Task is 8 dwords");
+static_assert(sizeof(TInstr) == 12 * sizeof(uint32_t), "TInstr is 12 dwords");
+__device__ __forceinline__ uint32_t task_words(const Task *tasks, int32_t ti, int lane)
+{
+	return lane < 8 ? global_of(reinterpret_cast<const uint32_t *>(tasks + ti))[lane] : 0u;
+}
+__device__ __forceinline__ uint32_t ins_words(const TInstr *ins, uint32_t vi, int lane)
+{
+	return lane < 12 ? global_of(reinterpret_cast<const uint32_t *>(ins + vi))[lane] : 0u;
+}
+template <typename S> __device__ __forceinline__ S unpack_words(uint32_t w)
+{
+	uint32_t d[sizeof(S) / sizeof(uint32_t)];
+#pragma unroll
+	for (int i = 0; i < (int)(sizeof(S) / sizeof(uint32_t)); i++)
+		d[i] = __builtin_amdgcn_readlane(w, i);
+	S s;
+	__builtin_memcpy(&s, d, sizeof(S));
+	return s;
+}
+
 template <typename VT> struct Load;
 template <> struct Load<float> {
-	__device__ static float at(const float *p, size_t i) { return p[i]; }
+	__device__ static float at(const float *p, size_t i) { return global_of(p)[i]; }
 };
 template <> struct Load<uint16_t> {
 	__device__ static float at(const uint16_t *p, size_t i)
 	{
-		return __uint_as_float(uint32_t(p[i]) << 16);
+		return __uint_as_float(uint32_t(global_of(p)[i]) << 16);
 	}
 };
 
@@ -146,9 +186,25 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const bool bvalid = b < B;    // B is a multiple of 4: all 4 or none
 	const VT *V = (const VT *)o.V;
 	const float *xs = w.space ? o.xT : o.x;
-	for (int32_t ti = w.t_begin + wave; ti < w.t_end; ti += nwaves) {
-		const Task T = o.tasks[ti];
-		const TInstr I = o.ins[T.vi];
+	// descriptor words of the current task, its instruction, and the next
+	// task (the next task's instruction is loaded when this one starts)
+	int32_t ti = w.t_begin + wave;
+	uint32_t tw_cur = 0, iw_cur = 0, tw_nxt = 0;
+	if (ti < w.t_end) {
+		tw_cur = task_words(o.tasks, ti, lane);
+		iw_cur = ins_words(o.ins, __builtin_amdgcn_readlane(tw_cur, 0), lane);
+		if (ti + nwaves < w.t_end)
+			tw_nxt = task_words(o.tasks, ti + nwaves, lane);
+	}
+	for (; ti < w.t_end; ti += nwaves) {
+		const Task T = unpack_words<Task>(tw_cur);
+		const TInstr I = unpack_words<TInstr>(iw_cur);
+		uint32_t iw_nxt = 0, tw_nn = 0;
+		if (ti + nwaves < w.t_end) {
+			iw_nxt = ins_words(o.ins, __builtin_amdgcn_readlane(tw_nxt, 0), lane);
+			if (ti + 2 * nwaves < w.t_end)
+				tw_nn = task_words(o.tasks, ti + 2 * nwaves, lane);
+		}
 		const float4 sc = bvalid ? ld4(o.scales + (size_t)I.force * B + b)
 					 : make_float4(0.f, 0.f, 0.f, 0.f);
 		// lane's vectors of the input block: element e at xb + e * B
@@ -175,12 +231,12 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					const uint32_t e = e0 + (uint32_t)(c * G + slot);
 					vl[c] = e < T.e1 && bvalid;
 					const uint32_t k = I.n_base + (e < T.e1 ? e : T.e0);
-					const uint2 en = o.ent[k];
+					const uint2 en = global_of(o.ent)[k];
 					// element column: compact index, or lane-list
 					// position for expanded element rows
 					col[c] = I.expanded ? k - I.n_base : en.y & kMask;
 					xin[c] = en.x;
-					outc[c] = o.keys[k];
+					outc[c] = global_of(o.keys)[k];
 					sgn[c] = sign_of(1.f, en.y);
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 				}
@@ -241,8 +297,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
 					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
-					pl = o.ent[jl];
-					const uint32_t prow = o.keys[jl];
+					pl = global_of(o.ent)[jl];
+					const uint32_t prow = global_of(o.keys)[jl];
 					issue(0u, B0{});
 					for (uint32_t t = 0; t < cnt; t += 2) {
 						if (t + 1 < cnt)
@@ -286,8 +342,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				for (int u = 0; u < 2; u++) {
 					const uint32_t j = j0 + (uint32_t)u * Gs;
 					const bool ok = j < T.je;
-					en[u] = o.ent[ok ? j : T.jb];
-					rw[u] = ok ? (int)o.keys[j] : -1;
+					en[u] = global_of(o.ent)[ok ? j : T.jb];
+					rw[u] = ok ? (int)global_of(o.keys)[j] : -1;
 				}
 #pragma unroll
 				for (int u = 0; u < 2; u++) {
@@ -319,8 +375,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
 					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
-					const uint2 pl = o.ent[jl];
-					const uint32_t prow = o.keys[jl];
+					const uint2 pl = global_of(o.ent)[jl];
+					const uint32_t prow = global_of(o.keys)[jl];
 					for (uint32_t t = 0; t < cnt; t++) {
 						const int row = (int)__builtin_amdgcn_readlane(prow, t);
 						if (row != cur) {
@@ -339,6 +395,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					add4_lds(&tile[((cur - w.po_begin) * nn + n) * pitch + q4], sc, a0);
 			}
 		}
+		tw_cur = tw_nxt;
+		iw_cur = iw_nxt;
+		tw_nxt = tw_nn;
 	}
 	__syncthreads();
 	float *yblk = ys + ((size_t)w.y_off + (size_t)nn * w.po_begin) * B + b0;
