@@ -339,6 +339,21 @@ int bacchus_hip_block_lanczos(bacchus_hip_batch_t *batch,
  * w ascending, V (may be NULL) eigenvector k in column k.  No GPU needed. */
 int bacchus_hip_symmetric_eigen(int64_t n, const double *A, double *w, double *V);
 
+/* Ceres's projections onto n stored vectors, in FP64 on the device.  V
+ * holds the vectors as device FP64 rows ([n][D], row i = v_i); the result is
+ * a host n x n row-major matrix, element [i][j] for j >= i mirrored into
+ * [j][i] as the reference fills it; on `stream` (NULL: the default stream),
+ * returning once the matrix is on the host.
+ *   norm_matrix replaces create_norm_matrix
+ *     (src/Ceres/norm_matrix/norm_matrix.c:4-28): N[i][j] = v_i . v_j
+ *   subspace_operator replaces create_subspace_operator
+ *     (src/Ceres/subspace_operator/subspace_operator.c:7-61):
+ *     M[i][j] = v_j . (H v_i), one matvec of op per vector (any storage
+ *     precision; the scales set on op apply) */
+int bacchus_hip_norm_matrix(const double *V, int64_t n, int64_t D, double *N, void *stream);
+int bacchus_hip_subspace_operator(bacchus_hip_operator_t *op, const double *V, int64_t n,
+				  double *M, void *stream);
+
 /* Stored-basis helpers of the learned start (no counterpart in the
  * reference).  U holds n FP32 vectors of dimension D ([n][D]).
  *   gram:      G[i][j] = U_i . U_j                  (host, n x n, FP64)

@@ -52,7 +52,8 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_basis_combine", "bacchus_hip_basis_transform",
     "bacchus_hip_write_evaluation_order", "bacchus_hip_block_lanczos",
     "bacchus_hip_symmetric_eigen", "bacchus_hip_operator_load_blocks",
-    "bacchus_hip_basis_project_static",
+    "bacchus_hip_basis_project_static", "bacchus_hip_norm_matrix",
+    "bacchus_hip_subspace_operator",
 )
 
 
@@ -190,6 +191,10 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_symmetric_eigen.argtypes = [i64, vp, vp, vp]
     lib.bacchus_hip_basis_transform.restype = c_int
     lib.bacchus_hip_basis_transform.argtypes = [vp, i64, i64, vp, i64, vp]
+    lib.bacchus_hip_norm_matrix.restype = c_int
+    lib.bacchus_hip_norm_matrix.argtypes = [vp, i64, i64, vp, vp]
+    lib.bacchus_hip_subspace_operator.restype = c_int
+    lib.bacchus_hip_subspace_operator.argtypes = [vp, vp, i64, vp, vp]
     _LIB = lib
     return lib
 
@@ -562,6 +567,33 @@ def _tensor(t, dtype, what):
     if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
         raise BacchusError(f"{what} must be a contiguous {dtype} device tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def norm_matrix(V, stream=None):
+    """N[i][j] = v_i . v_j (FP64, host (n, n)) for n float64 device vectors
+    V of shape (n, D), as create_norm_matrix (src/Ceres/norm_matrix/
+    norm_matrix.c:4-28)."""
+    import torch
+    n, D = V.shape
+    N = np.zeros((n, n))
+    _check(library().bacchus_hip_norm_matrix(_tensor(V, torch.float64, "V"), n, D, _ptr(N),
+                                              _stream_handle(stream)))
+    return N
+
+
+def subspace_operator(operator, V, stream=None):
+    """M[i][j] = v_j . (H v_i) (FP64, host (n, n)) for n float64 device
+    vectors V of shape (n, D), one FP64-interface matvec of `operator` per
+    vector, as create_subspace_operator (src/Ceres/subspace_operator/
+    subspace_operator.c:7-61)."""
+    import torch
+    n, D = V.shape
+    if D != operator.dimension:
+        raise BacchusError(f"vectors have {D} components, the operator {operator.dimension}")
+    M = np.zeros((n, n))
+    _check(library().bacchus_hip_subspace_operator(operator.handle, _tensor(V, torch.float64, "V"),
+                                                    n, _ptr(M), _stream_handle(stream)))
+    return M
 
 
 def basis_gram(U):

@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["operator.hip", "np.hip", "files.hip", "lanczos.hip", "batch.hip", "block.hip", "learned.hip", "mlp.hip"]
+HIP_SOURCES = ["operator.hip", "np.hip", "files.hip", "lanczos.hip", "batch.hip", "block.hip", "learned.hip", "mlp.hip", "subspace.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off on this target

@@ -164,3 +164,36 @@ def test_subspace_matrices_match_dense_projection(tmp_path):
     raw = open(path, "rb").read()
     assert raw[:6] == b"\x93NUMPY" and raw[6] == 1 and b"'<f8'" in raw[:128]
     assert np.array_equal(np.load(path, allow_pickle=False), M)
+
+
+@pytest.mark.parametrize("name", ["he4_nmax4", "he4_nmax2_3b"])
+def test_fp64_subspace_matrices_match_oracle(name):
+    """The FP64 form of the Ceres step (bacchus_hip_subspace_operator,
+    bacchus_hip_norm_matrix): float64 vectors, an FP64 operator with scales
+    other than 1, against the oracle's matvecs and numpy's dot products to
+    1e-12; the matrices exactly symmetric as the reference fills them."""
+    print(f"case FP64 subspace matrices {name}", flush=True)
+    torch = torch_dev()
+    from jupiterncsm_amd import subspace
+    from jupiterncsm_amd.inputs import SystemConfig
+    from oracle import oracle as O
+    cfg = SYSTEMS.get(name) or SystemConfig(2, 2, 2, max_body=3, seed=5, sigma3=0.3)
+    inp = build(cfg)
+    rng = np.random.default_rng(5)
+    n = 7
+    V = rng.standard_normal((n, inp.dimension))
+    Vd = torch.as_tensor(V, device="cuda")
+    scales = (1.0, 0.85, 1.15)
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.FP64) as op:
+        M = subspace.subspace_operator(op, Vd, scales)
+    N = subspace.norm_matrix(Vd)
+    p = O.Problem.from_inputs(inp)
+    HV = np.stack([O.matvec(p, v, scales=scales) for v in V], 0)
+    M_ref = np.empty((n, n))
+    for i in range(n):
+        for j in range(i, n):
+            M_ref[i, j] = M_ref[j, i] = V[j] @ HV[i]
+    assert np.abs(M - M_ref).max() <= 1e-12 * np.abs(M_ref).max()
+    assert np.array_equal(M, M.T) and np.array_equal(N, N.T)
+    assert np.abs(N - V @ V.T).max() <= 1e-12 * np.abs(V @ V.T).max()
+
