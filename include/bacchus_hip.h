@@ -276,6 +276,23 @@ int bacchus_hip_batch_autotune(bacchus_hip_batch_t *batch, int32_t repetitions,
 int bacchus_hip_batch_stats(const bacchus_hip_batch_t *batch, double *bytes,
 			    double *flops, int64_t *num_items);
 
+/* The matrix-core form of the batch's P-form instructions (P, and N in the
+ * transposed [neutron][proton] space; no counterpart in the reference):
+ * each is the product of its list, as a matrix with the summed signed
+ * elements, with a dense panel of (states x vectors) columns, stored as
+ * 16 x 4 tiles and run as v_mfma_f32_16x16x4f32 after the tile launch.
+ * Off when BACCHUS_HIP_BATCH_MFMA=0 is set at batch creation; instructions
+ * with a nonzero element below 2e-9 in magnitude, and operators while a
+ * scale is nonzero and below 1e-3 in magnitude, stay on the vector path, so
+ * the 1e-12 element skip of matrix_vector_multiplication.c:50-53 and
+ * :105-108 never applies to a matrix-core product.  kgroups = 16 x 4 tiles;
+ * mfma_per_matvec = MFMA instructions of one batched matvec with every
+ * operator on; fill = list entries / (64 kgroups); operators_on =
+ * operators whose P-form instructions run on the matrix cores now. */
+int bacchus_hip_batch_mfma_stats(const bacchus_hip_batch_t *batch, int64_t *kgroups,
+				 double *mfma_per_matvec, double *fill,
+				 int32_t *operators_on);
+
 /* One Lanczos recurrence per (operator, vector) in lockstep: full CGS2
  * reorthogonalization and all dot products in FP64, the matvec batched.
  * Each iteration is one captured hipGraph when use_graph != 0. */

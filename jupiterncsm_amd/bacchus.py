@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_write_evaluation_order", "bacchus_hip_block_lanczos",
     "bacchus_hip_symmetric_eigen", "bacchus_hip_operator_load_blocks",
     "bacchus_hip_basis_project_static", "bacchus_hip_norm_matrix",
-    "bacchus_hip_subspace_operator",
+    "bacchus_hip_subspace_operator", "bacchus_hip_batch_mfma_stats",
 )
 
 
@@ -171,6 +171,11 @@ def library() -> ctypes.CDLL:
     lib.bacchus_hip_batch_stats.restype = c_int
     lib.bacchus_hip_batch_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+    lib.bacchus_hip_batch_mfma_stats.restype = c_int
+    lib.bacchus_hip_batch_mfma_stats.argtypes = [vp, ctypes.POINTER(i64),
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_int32)]
     lib.bacchus_hip_batch_lanczos.restype = c_int
     lib.bacchus_hip_batch_lanczos.argtypes = [vp, ctypes.POINTER(_BatchLanczosSettings), vp, vp,
                                               ctypes.POINTER(_LanczosResult)]
@@ -473,6 +478,17 @@ class Batch:
         _check(library().bacchus_hip_batch_stats(self.handle, ctypes.byref(b), ctypes.byref(f),
                                                  ctypes.byref(w)))
         return {"algorithmic_bytes": b.value, "flops": f.value, "work_items": w.value}
+
+    def mfma_stats(self) -> dict:
+        """The matrix-core form of the P-form instructions: its 16 x 4 A
+        tiles, MFMA instructions per batched matvec, the share of tile
+        slots that hold a list entry, and the operators it runs for now."""
+        k, on = ctypes.c_int64(), ctypes.c_int32()
+        m, fill = ctypes.c_double(), ctypes.c_double()
+        _check(library().bacchus_hip_batch_mfma_stats(self.handle, ctypes.byref(k), ctypes.byref(m),
+                                                      ctypes.byref(fill), ctypes.byref(on)))
+        return {"kgroups": k.value, "mfma_per_matvec": m.value, "fill": fill.value,
+                "operators_on": on.value}
 
     def lanczos(self, max_num_iterations, num_eigenvalues=3, tolerance=0.0, check_every=5,
                 use_graph=True, seed=1, start_vectors=None, ritz_vectors=None, num_ritz=1):

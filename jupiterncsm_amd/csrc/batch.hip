@@ -45,7 +45,40 @@ struct BatchOp {
 	const float *scales;  // [4][B]: row f = scale of force class f
 	int32_t B;            // vectors, padded to a multiple of 4 (row stride)
 	int32_t precision;
+	// P-form virtual instructions applied by batch_mfma_kernel (nonzero
+	// entry per instruction) and whether that path is on for this launch
+	const uint32_t *mfma_vi;
+	int32_t mfma;
+	int32_t pad;
 };
+
+// --- the matrix-core form of P-form instructions ---
+//
+// A P instruction (and an N instruction in the transposed space) with list
+// entries (p_in -> p_out, element m, sign) is, over the flattened columns
+// (state n, vector b) of the batched vectors [state][vector],
+//   Y[p_out][(n, b)] += s_b A[p_out][p_in] X[p_in][(n, b)],
+//   A[p_out][p_in] = sum of sign V[m] over the entries connecting them,
+// a product of a sparse matrix with a dense panel of n_states x B columns.
+// A is stored as 16 x 4 tiles: for every 16-row group of an output block,
+// the input rows its entries read, four at a time ("K-groups"), with the
+// 64 values in v_mfma_f32_16x16x4f32 operand order (lane l holds
+// A[l % 16][l / 16]).  The scale s_b of the instruction's force class
+// multiplies the X operand.
+struct MfmaUnit {
+	int64_t y_off;    // block offset in y (yT in the transposed space), states
+	int32_t op;       // operator of the batch
+	int32_t space;
+	int32_t nrows;    // rows of the block in its space
+	int32_t ncols;    // states per row
+	int32_t r0;       // first row of the group
+	int32_t g[4];     // K-groups of force class f: [g[f - 1], g[f])
+};
+struct MfmaKGroup {
+	int64_t x_off;    // input block offset, states
+	int32_t rows[4];  // input rows of A's four columns
+};
+constexpr int kMfmaPanels = 4;  // 16-column panels per wave
 
 struct BatchItem {
 	int32_t op;
@@ -319,6 +352,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				else
 					rows_pass(std::false_type{});
 			}
+		} else if (I.type == 2 && o.mfma && global_of(o.mfma_vi)[T.vi]) {
+			// applied on the matrix cores by batch_mfma_kernel
 		} else if (I.type == 2 && 2 * (T.e1 - T.e0) <= (uint32_t)G) {
 			// narrow rows: slots of C >= (columns) lanes groups, C a power
 			// of two, G / C of them; slot g walks entries g, g + G / C, ...
@@ -450,6 +485,82 @@ __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 	}
 }
 
+typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
+
+// The P-form instructions of the operators with the matrix-core path on,
+// after batch_kernel (which skips them) and before the transposed blocks
+// are added back.  One wave per item {unit, first panel}: kMfmaPanels
+// panels of 16 flattened columns (n, b) of the unit's 16 rows, summed over
+// all its K-groups with v_mfma_f32_16x16x4f32 (lane l: A[l % 16][l / 16],
+// X[l / 16][l % 16]; result rows 4 (l / 16) + j, column l % 16), then added
+// to y.  A unit is the only writer of its rows in this launch.
+__global__ __launch_bounds__(256) void batch_mfma_kernel(
+	const BatchOp *__restrict__ ops, const MfmaUnit *__restrict__ units,
+	const MfmaKGroup *__restrict__ kgroups, const float *__restrict__ A,
+	const int2 *__restrict__ items, int nitems)
+{
+	const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+	if (it >= nitems)
+		return;
+	const int lane = threadIdx.x & 63;
+	const int2 item = items[it];
+	const MfmaUnit u = units[item.x];
+	const BatchOp o = ops[u.op];
+	if (!o.mfma)
+		return;
+	const int Bp = o.B;
+	const int64_t width = (int64_t)u.ncols * Bp;  // floats per row
+	const float *xs = u.space ? o.xT : o.x;
+	float *ys = u.space ? o.yT : o.y;
+	const int kq = lane >> 4;
+	int64_t col[kMfmaPanels];
+	bool cv[kMfmaPanels];
+	mfma_f32x4 acc[kMfmaPanels];
+#pragma unroll
+	for (int p = 0; p < kMfmaPanels; p++) {
+		col[p] = (int64_t)(item.y + p) * 16 + (lane & 15);
+		cv[p] = col[p] < width;
+		acc[p] = mfma_f32x4{0.f, 0.f, 0.f, 0.f};
+	}
+	// K-groups by force class, the class's scale on the X operand
+	int g = u.g[0];
+	for (int f = 1; f <= 3; f++) {
+		const int gend = u.g[f];
+		if (g >= gend)
+			continue;
+		float sc[kMfmaPanels];
+#pragma unroll
+		for (int p = 0; p < kMfmaPanels; p++)
+			sc[p] = global_of(o.scales)[(size_t)f * Bp + (int)(col[p] % Bp)];
+		for (; g < gend; g++) {
+			const MfmaKGroup K = kgroups[g];
+			const float a = A[(size_t)g * 64 + lane];
+			const int row = kq == 0 ? K.rows[0] : kq == 1 ? K.rows[1] : kq == 2 ? K.rows[2] : K.rows[3];
+			const float *xrow = xs + (size_t)(K.x_off + (int64_t)row * u.ncols) * Bp;
+			float bv[kMfmaPanels];
+#pragma unroll
+			for (int p = 0; p < kMfmaPanels; p++)
+				bv[p] = cv[p] ? global_of(xrow)[col[p]] * sc[p] : 0.f;
+#pragma unroll
+			for (int p = 0; p < kMfmaPanels; p++)
+				acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[p], acc[p], 0, 0, 0);
+		}
+	}
+#pragma unroll
+	for (int p = 0; p < kMfmaPanels; p++) {
+		if (!cv[p])
+			continue;
+#pragma unroll
+		for (int j = 0; j < 4; j++) {
+			const int row = u.r0 + 4 * kq + j;
+			if (row < u.nrows) {
+				float *dst = ys + (size_t)(u.y_off + (int64_t)row * u.ncols) * Bp + col[p];
+				*dst += acc[p][j];
+			}
+		}
+	}
+}
+
 // dst[r][b] = src[r][b] (b < B), 0 for the padding vectors b in [B, Bp)
 __global__ void pad_cols(const float *__restrict__ src, int64_t D, int B, int Bp,
 			 float *__restrict__ dst)
@@ -523,6 +634,20 @@ struct bacchus_hip_batch {
 	// transposed-space instructions)
 	std::vector<float *> d_xT, d_yT;
 	double bytes = 0, flops = 0;
+	// matrix-core path of the P-form instructions (batch_mfma_kernel):
+	// off with BACCHUS_HIP_BATCH_MFMA=0 at creation; per operator, on
+	// while its scales allow it (mfma_scales_ok)
+	bool mfma_enabled = false;
+	std::vector<uint32_t *> d_mfma_vi;
+	std::vector<char> mfma_has, mfma_scales_ok;
+	MfmaUnit *d_mfma_units = nullptr;
+	MfmaKGroup *d_mfma_kg = nullptr;
+	float *d_mfma_a = nullptr;
+	int2 *d_mfma_items = nullptr;
+	int64_t mfma_items = 0;
+	int64_t mfma_kgroups = 0;    // K-groups (one MFMA per 16 columns each)
+	double mfma_entries = 0;     // list entries they hold
+	double mfma_columns = 0;     // sum over units of K-groups x columns / 16
 
 	void release_plan()
 	{
@@ -553,6 +678,12 @@ struct bacchus_hip_batch {
 		(void)hipFree(d_ops);
 		(void)hipFree(d_next);
 		(void)hipFree(workspace);
+		for (auto *p : d_mfma_vi)
+			(void)hipFree(p);
+		(void)hipFree(d_mfma_units);
+		(void)hipFree(d_mfma_kg);
+		(void)hipFree(d_mfma_a);
+		(void)hipFree(d_mfma_items);
 	}
 	// device workspace of the block solver, kept between calls (grown on
 	// demand); calls on one batch must not overlap
@@ -567,6 +698,202 @@ int upload_ops(bacchus_hip_batch_t *bt)
 {
 	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_ops, bt->h_ops.data(),
 				    bt->h_ops.size() * sizeof(BatchOp), hipMemcpyHostToDevice));
+	return 0;
+}
+
+// Scales with which a P-form instruction may run on the matrix cores: the
+// reference skips a product when the (Vulcan-combined) element s V has
+// |s V| < 1e-12 (matrix_vector_multiplication.c:50-53, :105-108), which a
+// shared A cannot do per vector.  The matrix-core path is therefore used
+// only where no product can be skipped: instructions whose nonzero elements
+// all have |V| >= kMfmaMinElement (build_mfma), under scales that are 0
+// or have |s| >= kMfmaMinScale (the products of a zero scale are zero
+// either way).
+constexpr float kMfmaMinElement = 2e-9f;
+constexpr float kMfmaMinScale = 1e-3f;
+
+bool mfma_scales_allowed(const std::vector<float> &sc)
+{
+	for (float s : sc)
+		if (s != 0.f && !(std::fabs(s) >= kMfmaMinScale))
+			return false;
+	return true;
+}
+
+// The matrix-core tables of every operator's P-form virtual instructions,
+// built on the host from the operator's CSR row pointers and its entries
+// and elements read back from the device.  Instructions with an element
+// below kMfmaMinElement stay in batch_kernel.
+int build_mfma(bacchus_hip_batch_t *bt)
+{
+	const char *env = getenv("BACCHUS_HIP_BATCH_MFMA");
+	bt->mfma_enabled = !(env && atoi(env) == 0);
+	const int nops = (int)bt->ops.size();
+	bt->d_mfma_vi.assign(nops, nullptr);
+	bt->mfma_has.assign(nops, 0);
+	bt->mfma_scales_ok.assign(nops, 1);
+	std::vector<MfmaUnit> units;
+	std::vector<MfmaKGroup> kgs;
+	std::vector<double> A;
+	double entries_used = 0;
+	for (int i = 0; i < nops; i++) {
+		const bacchus_hip_operator_t *op = bt->ops[i];
+		const int64_t nb = (int64_t)op->block_nn.size();
+		std::vector<uint32_t> elig(std::max<size_t>(1, op->h_tinstr.size()), 0);
+		for (int64_t slot = 0; bt->mfma_enabled && slot < 2 * nb; slot++) {
+			const int64_t b = slot % nb;
+			const int space = slot >= nb;
+			const int64_t nrows = space ? op->block_nn[b] : op->block_np[b];
+			const int64_t ncols = space ? op->block_np[b] : op->block_nn[b];
+			struct PInstr {
+				int32_t v;
+				std::vector<uint2> ent;
+				std::vector<float> val;
+			};
+			std::vector<PInstr> pis;
+			for (int32_t v = op->blk_vi[slot]; v < op->blk_vi[slot + 1]; v++) {
+				const TInstr &t = op->h_tinstr[v];
+				if (t.type != 2 || t.nn_in != ncols)
+					continue;
+				const uint32_t e0 = op->h_rowptr[t.p_rows];
+				const uint32_t e1 = op->h_rowptr[t.p_rows + nrows];
+				if (e1 <= e0)
+					continue;
+				PInstr pi;
+				pi.v = v;
+				pi.ent.resize(e1 - e0);
+				BACCHUS_HIP_CHECK(hipMemcpy(pi.ent.data(), op->d_entries + e0,
+							    (e1 - e0) * sizeof(uint2), hipMemcpyDeviceToHost));
+				uint32_t mmax = 0;
+				for (const uint2 &en : pi.ent)
+					mmax = std::max(mmax, en.y & kMask);
+				pi.val.resize((size_t)mmax + 1);
+				if (op->precision == BACCHUS_HIP_FP32) {
+					BACCHUS_HIP_CHECK(hipMemcpy(pi.val.data(), (const float *)op->d_v + t.v_off,
+								    pi.val.size() * sizeof(float),
+								    hipMemcpyDeviceToHost));
+				} else {
+					std::vector<uint16_t> h(pi.val.size());
+					BACCHUS_HIP_CHECK(hipMemcpy(h.data(), (const uint16_t *)op->d_v + t.v_off,
+								    h.size() * sizeof(uint16_t),
+								    hipMemcpyDeviceToHost));
+					for (size_t k = 0; k < h.size(); k++) {
+						const uint32_t u = uint32_t(h[k]) << 16;
+						memcpy(&pi.val[k], &u, 4);
+					}
+				}
+				bool ok = true;
+				for (const uint2 &en : pi.ent) {
+					const float a = std::fabs(pi.val[en.y & kMask]);
+					if (a != 0.f && !(a >= kMfmaMinElement))
+						ok = false;
+				}
+				if (!ok)
+					continue;
+				elig[v] = 1;
+				pis.push_back(std::move(pi));
+			}
+			std::stable_sort(pis.begin(), pis.end(), [&](const PInstr &x, const PInstr &y) {
+				return op->h_tinstr[x.v].force < op->h_tinstr[y.v].force;
+			});
+			for (int64_t r0 = 0; r0 < nrows && !pis.empty(); r0 += 16) {
+				const int64_t r1 = std::min<int64_t>(nrows, r0 + 16);
+				MfmaUnit un{};
+				un.y_off = op->block_off[b];
+				un.op = i;
+				un.space = space;
+				un.nrows = (int32_t)nrows;
+				un.ncols = (int32_t)ncols;
+				un.r0 = (int32_t)r0;
+				un.g[0] = (int32_t)kgs.size();
+				int fcur = 1;
+				for (const PInstr &pi : pis) {
+					const TInstr &t = op->h_tinstr[pi.v];
+					const uint32_t base = op->h_rowptr[t.p_rows];
+					std::vector<uint32_t> cols;
+					for (int64_t r = r0; r < r1; r++)
+						for (uint32_t e = op->h_rowptr[t.p_rows + r];
+						     e < op->h_rowptr[t.p_rows + r + 1]; e++)
+							cols.push_back(pi.ent[e - base].x);
+					if (cols.empty())
+						continue;
+					std::sort(cols.begin(), cols.end());
+					cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+					// pis are ordered by force class
+					for (; fcur < t.force; fcur++)
+						un.g[fcur] = (int32_t)kgs.size();
+					const size_t first = kgs.size();
+					for (size_t k0 = 0; k0 < cols.size(); k0 += 4) {
+						MfmaKGroup kg{};
+						kg.x_off = t.x_off;
+						for (size_t q = 0; q < 4; q++)
+							kg.rows[q] = (int32_t)cols[std::min(k0 + q, cols.size() - 1)];
+						kgs.push_back(kg);
+					}
+					A.resize(kgs.size() * 64, 0.0);
+					for (int64_t r = r0; r < r1; r++)
+						for (uint32_t e = op->h_rowptr[t.p_rows + r];
+						     e < op->h_rowptr[t.p_rows + r + 1]; e++) {
+							const uint2 en = pi.ent[e - base];
+							const size_t k = (size_t)(std::lower_bound(cols.begin(), cols.end(), en.x) -
+										  cols.begin());
+							const double v = pi.val[en.y & kMask];
+							A[(first + k / 4) * 64 + (size_t)(r - r0) + 16 * (k % 4)] +=
+								(en.y & 0x80000000u) ? -v : v;
+							entries_used += 1;
+						}
+				}
+				for (; fcur <= 3; fcur++)
+					un.g[fcur] = (int32_t)kgs.size();
+				if (un.g[3] > un.g[0])
+					units.push_back(un);
+			}
+		}
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_vi[i], elig.size() * sizeof(uint32_t)));
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_vi[i], elig.data(), elig.size() * sizeof(uint32_t),
+					    hipMemcpyHostToDevice));
+		bt->mfma_has[i] = std::any_of(elig.begin(), elig.end(), [](uint32_t e) { return e != 0; });
+		bt->h_ops[i].mfma_vi = bt->d_mfma_vi[i];
+		bt->h_ops[i].mfma = bt->mfma_has[i] && bt->mfma_scales_ok[i];
+	}
+	if (kgs.size() > (size_t)INT32_MAX)
+		return BACCHUS_FAIL("more than 2^31 matrix-core K-groups");
+	// items: every unit in runs of kMfmaPanels panels, heaviest unit first
+	std::vector<std::pair<int32_t, int2>> items;
+	double mfmas = 0;
+	for (size_t k = 0; k < units.size(); k++) {
+		const MfmaUnit &u = units[k];
+		const int64_t panels = ((int64_t)u.ncols * bt->Bp[u.op] + 15) / 16;
+		mfmas += double(u.g[3] - u.g[0]) * double(panels);
+		for (int64_t p0 = 0; p0 < panels; p0 += kMfmaPanels)
+			items.push_back({u.g[3] - u.g[0], make_int2((int)k, (int)p0)});
+	}
+	std::stable_sort(items.begin(), items.end(),
+			 [](const std::pair<int32_t, int2> &a, const std::pair<int32_t, int2> &b) {
+				 return a.first > b.first;
+			 });
+	std::vector<int2> it(items.size());
+	for (size_t k = 0; k < items.size(); k++)
+		it[k] = items[k].second;
+	std::vector<float> Af(A.begin(), A.end());
+	bt->mfma_items = (int64_t)it.size();
+	bt->mfma_kgroups = (int64_t)kgs.size();
+	bt->mfma_entries = entries_used;
+	bt->mfma_columns = mfmas;
+	if (!it.empty()) {
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_units, units.size() * sizeof(MfmaUnit)));
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_kg, kgs.size() * sizeof(MfmaKGroup)));
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_a, Af.size() * sizeof(float)));
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_items, it.size() * sizeof(int2)));
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_units, units.data(), units.size() * sizeof(MfmaUnit),
+					    hipMemcpyHostToDevice));
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_kg, kgs.data(), kgs.size() * sizeof(MfmaKGroup),
+					    hipMemcpyHostToDevice));
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_a, Af.data(), Af.size() * sizeof(float),
+					    hipMemcpyHostToDevice));
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_items, it.data(), it.size() * sizeof(int2),
+					    hipMemcpyHostToDevice));
+	}
 	return 0;
 }
 
@@ -708,6 +1035,10 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 		hipLaunchKernelGGL(batch_kernel, dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
 				   bt->d_ops, bt->d_items, bt->d_qbegin, bt->nq, bt->d_next);
 	}
+	if (bt->mfma_items)
+		hipLaunchKernelGGL(batch_mfma_kernel, dim3((unsigned)((bt->mfma_items + 3) / 4)), dim3(256),
+				   0, s, bt->d_ops, bt->d_mfma_units, bt->d_mfma_kg, bt->d_mfma_a,
+				   bt->d_mfma_items, (int)bt->mfma_items);
 	for (size_t i = 0; i < bt->ops.size(); i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
 		if (op->has_transposed)
@@ -859,6 +1190,8 @@ int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops, const int32_t *
 	if (hipMalloc(&bt->d_ops, num_ops * sizeof(BatchOp)) != hipSuccess ||
 	    hipMalloc(&bt->d_next, kXcds * sizeof(int)) != hipSuccess)
 		return fail(BACCHUS_FAIL("hipMalloc of the descriptors failed"));
+	if (int rc = build_mfma(bt))
+		return fail(rc);
 	if (int rc = build_plan(bt))
 		return fail(rc);
 	if (int rc = use_internal(bt))
@@ -882,6 +1215,12 @@ int bacchus_hip_batch_set_scales(bacchus_hip_batch_t *bt, int32_t i, const doubl
 			sc[(size_t)(f + 1) * Bp + b] = (float)scales[(size_t)b * 3 + f];
 	BACCHUS_HIP_CHECK(hipMemcpy(bt->d_scales[i], sc.data(), sc.size() * sizeof(float),
 				    hipMemcpyHostToDevice));
+	bt->mfma_scales_ok[i] = mfma_scales_allowed(sc);
+	const int32_t on = bt->mfma_has[i] && bt->mfma_scales_ok[i];
+	if (on != bt->h_ops[i].mfma) {
+		bt->h_ops[i].mfma = on;
+		return upload_ops(bt);
+	}
 	return 0;
 }
 
@@ -942,6 +1281,26 @@ int bacchus_hip_batch_stats(const bacchus_hip_batch_t *bt, double *bytes, double
 		*flops = bt->flops;
 	if (num_items)
 		*num_items = bt->num_items;
+	return 0;
+}
+
+int bacchus_hip_batch_mfma_stats(const bacchus_hip_batch_t *bt, int64_t *kgroups,
+				 double *mfma_per_matvec, double *fill, int32_t *operators_on)
+{
+	if (!bt)
+		return BACCHUS_FAIL("NULL argument");
+	if (kgroups)
+		*kgroups = bt->mfma_kgroups;
+	if (mfma_per_matvec)
+		*mfma_per_matvec = bt->mfma_columns;
+	if (fill)
+		*fill = bt->mfma_kgroups ? bt->mfma_entries / (64.0 * double(bt->mfma_kgroups)) : 0.0;
+	if (operators_on) {
+		int32_t on = 0;
+		for (const BatchOp &o : bt->h_ops)
+			on += o.mfma != 0;
+		*operators_on = on;
+	}
 	return 0;
 }
 

@@ -240,3 +240,92 @@ def test_batched_tiny_element_skip(case):
         err = np.abs(Y[:, b] - want[:, b]).max() / np.abs(want[:, b]).max()
         print("batched vs hand-computed, vector", b, err, flush=True)
         assert err <= 2e-6
+
+
+def batched_y(inps, ops, counts, scales, xs_host, mfma):
+    """y of one batched matvec, the batch created with the matrix-core path
+    of the P-form instructions on or off; returns (ys, mfma_stats)."""
+    import os
+    torch = torch_dev()
+    saved = os.environ.get("BACCHUS_HIP_BATCH_MFMA")
+    os.environ["BACCHUS_HIP_BATCH_MFMA"] = "1" if mfma else "0"
+    try:
+        bt = bacchus.Batch(ops, counts)
+    finally:
+        if saved is None:
+            del os.environ["BACCHUS_HIP_BATCH_MFMA"]
+        else:
+            os.environ["BACCHUS_HIP_BATCH_MFMA"] = saved
+    with bt:
+        for k, sc in enumerate(scales):
+            bt.set_scales(k, sc)
+        xs = [torch.as_tensor(x, dtype=torch.float32, device="cuda").contiguous() for x in xs_host]
+        ys = [torch.full_like(x, 7.0) for x in xs]
+        bt.matvec(xs, ys)
+        torch.cuda.synchronize()
+        stats = bt.mfma_stats()
+        return [y.cpu().numpy().astype(np.float64) for y in ys], stats
+
+
+@pytest.mark.parametrize("precision,tol", [(bacchus.FP32, 2e-6), (bacchus.BF16, 2e-2)])
+def test_batched_mfma_path_matches_oracle_and_vector_path(precision, tol):
+    """The P-form instructions (P, and N in the transposed space) on
+    v_mfma_f32_16x16x4f32: three operators of uneven dimension (2-body and
+    3-body), vector counts 3, 5, 2 (padded to 4, 8, 4), per-vector scales;
+    against the FP64 oracle and against the same batch with the matrix-core
+    path off (both FP32 products and sums, so within 1e-5)."""
+    print(f"case batch mfma precision {precision}", flush=True)
+    systems = [SYSTEMS["he4_nmax4"], SYSTEMS["he4_nmax6"], SMALL_3B]
+    counts = [3, 5, 2]
+    inps = [build(s) for s in systems]
+    scales = [scales_for(B, 20 + k) for k, B in enumerate(counts)]
+    xs = [np.random.default_rng(30 + k).standard_normal((i.dimension, B))
+          for k, (i, B) in enumerate(zip(inps, counts))]
+    ops = [bacchus.Operator.from_inputs(i, precision=precision) for i in inps]
+    try:
+        y_on, st_on = batched_y(inps, ops, counts, scales, xs, True)
+        y_off, st_off = batched_y(inps, ops, counts, scales, xs, False)
+    finally:
+        for o in ops:
+            o.close()
+    print("mfma stats", st_on, st_off, flush=True)
+    assert st_on["operators_on"] == 3 and st_on["kgroups"] > 0
+    assert 0.0 < st_on["fill"] <= 1.0 and st_on["mfma_per_matvec"] >= st_on["kgroups"]
+    assert st_off["operators_on"] == 0 and st_off["kgroups"] == 0
+    for k, (inp, B) in enumerate(zip(inps, counts)):
+        p = O.Problem.from_inputs(inp)
+        ref = np.stack([O.matvec(p, xs[k][:, b], scales=scales[k][b]) for b in range(B)], 1)
+        e_on = np.abs(y_on[k] - ref).max() / np.abs(ref).max()
+        e_ab = np.abs(y_on[k] - y_off[k]).max() / np.abs(y_off[k]).max()
+        print(f"operator {k}: vs oracle {e_on:.2e}, vs vector path {e_ab:.2e}", flush=True)
+        assert e_on <= tol
+        assert e_ab <= 1e-5
+
+
+def test_batched_mfma_path_off_for_small_scales():
+    """A scale that is nonzero and below 1e-3 in magnitude turns the
+    matrix-core path off for that operator (the 1e-12 skip on s V could then
+    act on a product), and the results stay those of the oracle; normal
+    scales turn it on again."""
+    print("case batch mfma scale gate", flush=True)
+    torch = torch_dev()
+    inp = build(SYSTEMS["he4_nmax6"])
+    B = 4
+    x = np.random.default_rng(41).standard_normal((inp.dimension, B))
+    p = O.Problem.from_inputs(inp)
+    with bacchus.Operator.from_inputs(inp, precision=bacchus.FP32) as op, \
+            bacchus.Batch([op], [B]) as bt:
+        xd = torch.as_tensor(x, dtype=torch.float32, device="cuda").contiguous()
+        for tiny, on in ((5e-4, 0), (1.0, 1), (0.0, 1)):
+            sc = scales_for(B, 42)
+            sc[2, 0] = tiny
+            bt.set_scales(0, sc)
+            assert bt.mfma_stats()["operators_on"] == on
+            yd = torch.zeros_like(xd)
+            bt.matvec([xd], [yd])
+            torch.cuda.synchronize()
+            y = yd.cpu().numpy().astype(np.float64)
+            ref = np.stack([O.matvec(p, x[:, b], scales=sc[b]) for b in range(B)], 1)
+            err = np.abs(y - ref).max() / np.abs(ref).max()
+            print(f"s = {tiny}: path on {on}, error {err:.2e}", flush=True)
+            assert err <= 2e-6
