@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -78,7 +79,13 @@ struct MfmaKGroup {
 	int64_t x_off;    // input block offset, states
 	int32_t rows[4];  // input rows of A's four columns
 };
-constexpr int kMfmaPanels = 4;  // 16-column panels per wave
+// one wave's work: K-groups [g0, g1) of a unit over one column chunk
+struct MfmaItem {
+	int32_t unit;
+	int32_t chunk;
+	int32_t g0, g1;
+};
+constexpr int kMfmaSplit = 32;  // K-groups per item at most
 
 struct BatchItem {
 	int32_t op;
@@ -487,78 +494,136 @@ __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 
 typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
 
-// The P-form instructions of the operators with the matrix-core path on,
-// after batch_kernel (which skips them) and before the transposed blocks
-// are added back.  One wave per item {unit, first panel}: kMfmaPanels
-// panels of 16 flattened columns (n, b) of the unit's 16 rows, summed over
-// all its K-groups with v_mfma_f32_16x16x4f32 (lane l: A[l % 16][l / 16],
-// X[l / 16][l % 16]; result rows 4 (l / 16) + j, column l % 16), then added
-// to y.  A unit is the only writer of its rows in this launch.
-__global__ __launch_bounds__(256) void batch_mfma_kernel(
-	const BatchOp *__restrict__ ops, const MfmaUnit *__restrict__ units,
-	const MfmaKGroup *__restrict__ kgroups, const float *__restrict__ A,
-	const int2 *__restrict__ items, int nitems)
+// One wave of batch_mfma_kernel: K-groups [g0, g1) of unit u over the
+// column chunk `chunk` of 64 NQ flattened columns (n, b) of its 16 rows,
+// with v_mfma_f32_16x16x4f32 (lane l: A[l % 16][l / 16], X[l / 16][l % 16];
+// result rows 4 (l / 16) + j, column l % 16).  The 4 NQ panels of a chunk
+// are interleaved: column j of panel (q, e) is column 64 q + 4 j + e of the
+// chunk, so lane l loads its four panels' X values of quad q as one float4
+// (columns 64 q + 4 (l % 16) + 0..3) and adds their results to y as one
+// float4.  The loads of K-group g + 1 are issued before the products of g.
+// Units whose K-groups are split over several items add atomically.
+template <int NQ>
+__device__ void mfma_item(const BatchOp &o, const MfmaUnit &u, int chunk, int g0, int g1,
+			  bool split, const MfmaKGroup *__restrict__ kgroups,
+			  const float *__restrict__ A)
 {
-	const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-	if (it >= nitems)
-		return;
 	const int lane = threadIdx.x & 63;
-	const int2 item = items[it];
-	const MfmaUnit u = units[item.x];
-	const BatchOp o = ops[u.op];
-	if (!o.mfma)
-		return;
 	const int Bp = o.B;
 	const int64_t width = (int64_t)u.ncols * Bp;  // floats per row
 	const float *xs = u.space ? o.xT : o.x;
 	float *ys = u.space ? o.yT : o.y;
 	const int kq = lane >> 4;
-	int64_t col[kMfmaPanels];
-	bool cv[kMfmaPanels];
-	mfma_f32x4 acc[kMfmaPanels];
+	// Bp is a multiple of 4: a quad of columns is all inside the row or
+	// all outside, and its 4 vectors are consecutive
+	int64_t c0[NQ];
+	bool cv[NQ];
+	mfma_f32x4 acc[4 * NQ];
 #pragma unroll
-	for (int p = 0; p < kMfmaPanels; p++) {
-		col[p] = (int64_t)(item.y + p) * 16 + (lane & 15);
-		cv[p] = col[p] < width;
-		acc[p] = mfma_f32x4{0.f, 0.f, 0.f, 0.f};
+	for (int q = 0; q < NQ; q++) {
+		c0[q] = (int64_t)chunk * 64 * NQ + 64 * q + 4 * (lane & 15);
+		cv[q] = c0[q] < width;
 	}
-	// K-groups by force class, the class's scale on the X operand
-	int g = u.g[0];
+#pragma unroll
+	for (int k = 0; k < 4 * NQ; k++)
+		acc[k] = mfma_f32x4{0.f, 0.f, 0.f, 0.f};
+	auto xrow_of = [&](int g) {
+		const MfmaKGroup K = kgroups[g];
+		const int row = kq == 0 ? K.rows[0] : kq == 1 ? K.rows[1] : kq == 2 ? K.rows[2] : K.rows[3];
+		return xs + (size_t)(K.x_off + (int64_t)row * u.ncols) * Bp;
+	};
+	// K-groups by force class, the class's scales on the X operand
 	for (int f = 1; f <= 3; f++) {
-		const int gend = u.g[f];
+		int g = max(g0, u.g[f - 1]);
+		const int gend = min(g1, u.g[f]);
 		if (g >= gend)
 			continue;
-		float sc[kMfmaPanels];
+		float4 sc[NQ];
 #pragma unroll
-		for (int p = 0; p < kMfmaPanels; p++)
-			sc[p] = global_of(o.scales)[(size_t)f * Bp + (int)(col[p] % Bp)];
+		for (int q = 0; q < NQ; q++)
+			sc[q] = ld4(o.scales + (size_t)f * Bp + (int)(c0[q] % Bp));
+		float a = A[(size_t)g * 64 + lane];
+		float4 xv[NQ];
+		{
+			const float *xr = xrow_of(g);
+#pragma unroll
+			for (int q = 0; q < NQ; q++)
+				xv[q] = cv[q] ? ld4(xr + c0[q]) : make_float4(0.f, 0.f, 0.f, 0.f);
+		}
 		for (; g < gend; g++) {
-			const MfmaKGroup K = kgroups[g];
-			const float a = A[(size_t)g * 64 + lane];
-			const int row = kq == 0 ? K.rows[0] : kq == 1 ? K.rows[1] : kq == 2 ? K.rows[2] : K.rows[3];
-			const float *xrow = xs + (size_t)(K.x_off + (int64_t)row * u.ncols) * Bp;
-			float bv[kMfmaPanels];
+			float an = 0.f;
+			float4 xn[NQ];
+			if (g + 1 < gend) {
+				an = A[(size_t)(g + 1) * 64 + lane];
+				const float *xr = xrow_of(g + 1);
 #pragma unroll
-			for (int p = 0; p < kMfmaPanels; p++)
-				bv[p] = cv[p] ? global_of(xrow)[col[p]] * sc[p] : 0.f;
+				for (int q = 0; q < NQ; q++)
+					xn[q] = cv[q] ? ld4(xr + c0[q]) : make_float4(0.f, 0.f, 0.f, 0.f);
+			}
 #pragma unroll
-			for (int p = 0; p < kMfmaPanels; p++)
-				acc[p] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[p], acc[p], 0, 0, 0);
+			for (int q = 0; q < NQ; q++) {
+				acc[4 * q + 0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xv[q].x * sc[q].x, acc[4 * q + 0], 0, 0, 0);
+				acc[4 * q + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xv[q].y * sc[q].y, acc[4 * q + 1], 0, 0, 0);
+				acc[4 * q + 2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xv[q].z * sc[q].z, acc[4 * q + 2], 0, 0, 0);
+				acc[4 * q + 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xv[q].w * sc[q].w, acc[4 * q + 3], 0, 0, 0);
+			}
+			if (g + 1 < gend) {
+				a = an;
+#pragma unroll
+				for (int q = 0; q < NQ; q++)
+					xv[q] = xn[q];
+			}
 		}
 	}
 #pragma unroll
-	for (int p = 0; p < kMfmaPanels; p++) {
-		if (!cv[p])
+	for (int q = 0; q < NQ; q++) {
+		if (!cv[q])
 			continue;
 #pragma unroll
 		for (int j = 0; j < 4; j++) {
 			const int row = u.r0 + 4 * kq + j;
-			if (row < u.nrows) {
-				float *dst = ys + (size_t)(u.y_off + (int64_t)row * u.ncols) * Bp + col[p];
-				*dst += acc[p][j];
+			if (row >= u.nrows)
+				continue;
+			float *dst = ys + (size_t)(u.y_off + (int64_t)row * u.ncols) * Bp + c0[q];
+			if (split) {
+				atomicAdd(dst + 0, acc[4 * q + 0][j]);
+				atomicAdd(dst + 1, acc[4 * q + 1][j]);
+				atomicAdd(dst + 2, acc[4 * q + 2][j]);
+				atomicAdd(dst + 3, acc[4 * q + 3][j]);
+			} else {
+				float4 v = *reinterpret_cast<float4 *>(dst);
+				v.x += acc[4 * q + 0][j];
+				v.y += acc[4 * q + 1][j];
+				v.z += acc[4 * q + 2][j];
+				v.w += acc[4 * q + 3][j];
+				*reinterpret_cast<float4 *>(dst) = v;
 			}
 		}
 	}
+}
+
+// The P-form instructions of the operators with the matrix-core path on,
+// after batch_kernel (which skips them) and before the transposed blocks
+// are added back; one wave per item, chunks of 128 columns for rows wider
+// than 64 columns, of 64 otherwise.
+__global__ __launch_bounds__(256) void batch_mfma_kernel(
+	const BatchOp *__restrict__ ops, const MfmaUnit *__restrict__ units,
+	const MfmaKGroup *__restrict__ kgroups, const float *__restrict__ A,
+	const MfmaItem *__restrict__ items, int nitems)
+{
+	const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+	if (it >= nitems)
+		return;
+	const MfmaItem item = items[it];
+	const MfmaUnit u = units[item.unit];
+	const BatchOp o = ops[u.op];
+	if (!o.mfma)
+		return;
+	const bool split = item.g0 > u.g[0] || item.g1 < u.g[3];
+	if ((int64_t)u.ncols * o.B > 64)
+		mfma_item<2>(o, u, item.chunk, item.g0, item.g1, split, kgroups, A);
+	else
+		mfma_item<1>(o, u, item.chunk, item.g0, item.g1, split, kgroups, A);
 }
 
 // dst[r][b] = src[r][b] (b < B), 0 for the padding vectors b in [B, Bp)
@@ -643,7 +708,7 @@ struct bacchus_hip_batch {
 	MfmaUnit *d_mfma_units = nullptr;
 	MfmaKGroup *d_mfma_kg = nullptr;
 	float *d_mfma_a = nullptr;
-	int2 *d_mfma_items = nullptr;
+	MfmaItem *d_mfma_items = nullptr;
 	int64_t mfma_items = 0;
 	int64_t mfma_kgroups = 0;    // K-groups (one MFMA per 16 columns each)
 	double mfma_entries = 0;     // list entries they hold
@@ -858,23 +923,41 @@ int build_mfma(bacchus_hip_batch_t *bt)
 	}
 	if (kgs.size() > (size_t)INT32_MAX)
 		return BACCHUS_FAIL("more than 2^31 matrix-core K-groups");
-	// items: every unit in runs of kMfmaPanels panels, heaviest unit first
-	std::vector<std::pair<int32_t, int2>> items;
+	// items: every unit in column chunks (128 columns for rows wider than
+	// 64 columns, 64 otherwise) and runs of at most kMfmaSplit K-groups,
+	// the longest runs first
+	std::vector<std::pair<int32_t, MfmaItem>> items;
 	double mfmas = 0;
 	for (size_t k = 0; k < units.size(); k++) {
 		const MfmaUnit &u = units[k];
-		const int64_t panels = ((int64_t)u.ncols * bt->Bp[u.op] + 15) / 16;
-		mfmas += double(u.g[3] - u.g[0]) * double(panels);
-		for (int64_t p0 = 0; p0 < panels; p0 += kMfmaPanels)
-			items.push_back({u.g[3] - u.g[0], make_int2((int)k, (int)p0)});
+		const int64_t width = (int64_t)u.ncols * bt->Bp[u.op];
+		const int nq = width > 64 ? 2 : 1;
+		const int64_t chunks = (width + 64 * nq - 1) / (64 * nq);
+		mfmas += double(u.g[3] - u.g[0]) * double(chunks * 4 * nq);
+		for (int64_t c = 0; c < chunks; c++)
+			for (int32_t g = u.g[0]; g < u.g[3]; g += kMfmaSplit) {
+				const int32_t g1 = std::min(u.g[3], g + kMfmaSplit);
+				items.push_back({(g1 - g) * nq, MfmaItem{(int32_t)k, (int32_t)c, g, g1}});
+			}
 	}
 	std::stable_sort(items.begin(), items.end(),
-			 [](const std::pair<int32_t, int2> &a, const std::pair<int32_t, int2> &b) {
+			 [](const std::pair<int32_t, MfmaItem> &a, const std::pair<int32_t, MfmaItem> &b) {
 				 return a.first > b.first;
 			 });
-	std::vector<int2> it(items.size());
+	std::vector<MfmaItem> it(items.size());
 	for (size_t k = 0; k < items.size(); k++)
 		it[k] = items[k].second;
+	if (getenv("BACCHUS_HIP_VERBOSE") && !units.empty()) {
+		int64_t kmax = 0, narrow = 0;
+		for (const MfmaUnit &u : units) {
+			kmax = std::max<int64_t>(kmax, u.g[3] - u.g[0]);
+			narrow += (int64_t)u.ncols * bt->Bp[u.op] <= 64;
+		}
+		fprintf(stderr, "[bacchus] matrix-core P-form path: %zu units (%lld with rows of <= 64 "
+			"columns), %zu K-groups (at most %lld per unit), %zu items, fill %.3f, %.0f MFMA "
+			"per matvec\n", units.size(), (long long)narrow, kgs.size(), (long long)kmax,
+			it.size(), kgs.empty() ? 0.0 : entries_used / (64.0 * kgs.size()), mfmas);
+	}
 	std::vector<float> Af(A.begin(), A.end());
 	bt->mfma_items = (int64_t)it.size();
 	bt->mfma_kgroups = (int64_t)kgs.size();
@@ -884,14 +967,14 @@ int build_mfma(bacchus_hip_batch_t *bt)
 		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_units, units.size() * sizeof(MfmaUnit)));
 		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_kg, kgs.size() * sizeof(MfmaKGroup)));
 		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_a, Af.size() * sizeof(float)));
-		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_items, it.size() * sizeof(int2)));
+		BACCHUS_HIP_CHECK(hipMalloc(&bt->d_mfma_items, it.size() * sizeof(MfmaItem)));
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_units, units.data(), units.size() * sizeof(MfmaUnit),
 					    hipMemcpyHostToDevice));
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_kg, kgs.data(), kgs.size() * sizeof(MfmaKGroup),
 					    hipMemcpyHostToDevice));
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_a, Af.data(), Af.size() * sizeof(float),
 					    hipMemcpyHostToDevice));
-		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_items, it.data(), it.size() * sizeof(int2),
+		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_mfma_items, it.data(), it.size() * sizeof(MfmaItem),
 					    hipMemcpyHostToDevice));
 	}
 	return 0;
