@@ -239,7 +239,7 @@ def main():
         log(f"{pname} storage: relative eigenvalue errors {by_precision[pname]}")
 
     # --- learned start: trained on batched solves of the family (untimed) -
-    model = fam = bt = mblock = None
+    model = fam = bt = mblock = block_mv = None
     train_s = 0.0
     if not args.no_learned:
         from jupiterncsm_amd import learned as L
@@ -249,9 +249,10 @@ def main():
                              num_eigenvalues=args.eigenvalues, target="band",
                              band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues),
                              band_references=args.block if args.block_mlp_steps else 0)
-        model = L.fit(sol, basis_size=64, steps=3000, target="band")
+        model = L.fit(sol, basis_size=64, steps=3000, target="band", precision=MLP_PRECISION)
         if args.block_mlp_steps:
-            mblock = L.fit_block(sol, basis_size=64, steps=args.block_mlp_steps)
+            mblock = L.fit_block(sol, basis_size=64, steps=args.block_mlp_steps,
+                                 precision=MLP_PRECISION)
             log(f"MLP start block: {args.block} networks, losses "
                 f"{min(mblock.train_losses):.2e}..{max(mblock.train_losses):.2e}")
         t1 = time.time()
@@ -263,7 +264,10 @@ def main():
         bt = bacchus.Batch([op32], [args.block])
         bt.set_scales(0, np.ones((args.block, 3)))
         tuned = bt.autotune(3)
-        log(f"block matvec ({args.block} vectors) autotuned: {tuned['ms']:.2f} ms")
+        block_mv = {"vectors": args.block, "ms": round(tuned["ms"], 4),
+                    "mfma": bt.mfma_stats()}
+        log(f"block matvec ({args.block} vectors) autotuned: {tuned['ms']:.2f} ms, "
+            f"matrix cores {block_mv['mfma']}")
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -472,6 +476,7 @@ def main():
             if fam is not None else None),
         "matvec_fp64_ms": round(mv_ms, 4),
         "matvec_fp32_ms": round(mv32_ms, 4),
+        "block_matvec": block_mv,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(args.workload)},
@@ -482,6 +487,9 @@ def main():
 
 
 FIXTURE_SEED = 20261204
+# the learned-start networks: BF16 weights, activations carried as two BF16
+# parts (bacchus_hip_mlp_set_precision)
+MLP_PRECISION = 1
 
 
 def start_vector(dimension, seed):
@@ -562,6 +570,21 @@ def small_systems(args, cores, cpu=True):
     return out
 
 
+def batch_without_mfma(ops, counts):
+    """A batch whose P-form instructions run on the vector path (the
+    matrix-core path is chosen when the batch is created)."""
+    from jupiterncsm_amd import bacchus
+    saved = os.environ.get("BACCHUS_HIP_BATCH_MFMA")
+    os.environ["BACCHUS_HIP_BATCH_MFMA"] = "0"
+    try:
+        return bacchus.Batch(ops, counts)
+    finally:
+        if saved is None:
+            del os.environ["BACCHUS_HIP_BATCH_MFMA"]
+        else:
+            os.environ["BACCHUS_HIP_BATCH_MFMA"] = saved
+
+
 def batched_sweep(args, emit=True):
     """64 Hamiltonians H(c1, c2) spread over the 2+2 systems at Nmax 4, 6,
     8: one batched Lanczos (all matvecs of an iteration in one persistent
@@ -602,6 +625,23 @@ def batched_sweep(args, emit=True):
     e1.record()
     torch.cuda.synchronize()
     bmv = e0.elapsed_time(e1) / 20
+    mfma = bt.mfma_stats()
+    # the same batch with the P-form instructions on the vector path
+    bt_v = batch_without_mfma(ops32, counts)
+    for k, s in enumerate(split):
+        bt_v.set_scales(k, L.scales_of(pts[s]))
+    bt_v.autotune(3)
+    for _ in range(3):
+        bt_v.matvec(xs, ys)
+    e0.record()
+    for _ in range(20):
+        bt_v.matvec(xs, ys)
+    e1.record()
+    torch.cuda.synchronize()
+    bmv_vector = e0.elapsed_time(e1) / 20
+    bt_v.close()
+    log(f"batched matvec {bmv:.4f} ms (P-form on the matrix cores: {mfma}), "
+        f"{bmv_vector:.4f} ms on the vector path")
     for _ in range(args.warmup):
         bt.lanczos(K, num_eigenvalues=nev, use_graph=True, seed=5)
     times = []
@@ -652,7 +692,10 @@ def batched_sweep(args, emit=True):
         "batched_matvec_algorithmic_gbs": round(st["algorithmic_bytes"] / bmv / 1e6, 1),
         "autotuned": tune,
         "max_rel_diff_vs_fp64_solves": max(errs),
-        "mfma": "not used: the batched products are sparse gathers (see design notes)",
+        "mfma": {"form": "P-form instructions (P, and N in the transposed space) as 16x4 "
+                         "tiles of their signed element matrix on v_mfma_f32_16x16x4f32 over "
+                         "(state x vector) panels; NP instructions on the vector path",
+                 **mfma, "batched_matvec_ms_vector_path": round(bmv_vector, 4)},
     }
     for o in ops32 + ops64:
         o.close()
@@ -684,7 +727,7 @@ def learned_start(args, emit=True):
     log(f"held-out references: {ref.seconds:.1f}s")
     sol = L.solve_family(op32, train, iterations=60, num_eigenvalues=1, target="ground")
     log(f"training solves: {sol.seconds:.1f}s")
-    model = L.fit(sol, basis_size=64, steps=6000, target="ground")
+    model = L.fit(sol, basis_size=64, steps=6000, target="ground", precision=MLP_PRECISION)
     log(f"MLP fit: loss {model.train_loss:.2e}, total {time.time() - t0:.1f}s")
     rows = []
     fallbacks = 0
@@ -715,7 +758,8 @@ def learned_start(args, emit=True):
         "data": "synthetic",
         "config": {"workload": "li6_nmax6_3b family H1 + c1 V2 + c2 V3, c ~ U[0.8,1.2]",
                    "train_points": len(train), "test_points": len(test), "basis": 64,
-                   "mlp": "2-256-256-256-64 GELU, BF16 weights, FP32 accumulate"},
+                   "mlp": "2-256-256-256-64 GELU, BF16 weights, inputs and activations as "
+                          "BF16 hi+lo pairs on the MFMA, FP32 accumulate"},
         "random_start_iterations": round(float(a[:, 2].mean()), 2),
         "learned_ms": round(float(a[:, 1].mean()) * 1e3, 2),
         "random_ms": round(float(a[:, 3].mean()) * 1e3, 2),

@@ -420,6 +420,16 @@ int bacchus_hip_mlp_train_step(bacchus_hip_mlp_t *mlp, const float *x,
 			       const float *t, int M, float lr,
 			       float *loss_host, void *stream);
 
+/* Precision of the MLP's GEMMs (default 0: every operand rounded to BF16).
+ * BACCHUS_HIP_MLP_SPLIT_ACTIVATIONS: inputs, activations and backward
+ * operands other than the weights are carried as two BF16 parts
+ * (v = hi + lo, 16 significant bits; one more MFMA per tile), the weights
+ * staying BF16.  BACCHUS_HIP_MLP_FP32_LAST: the output layer's FP32 master
+ * weights are used, as two BF16 parts. */
+#define BACCHUS_HIP_MLP_SPLIT_ACTIVATIONS 1
+#define BACCHUS_HIP_MLP_FP32_LAST 2
+int bacchus_hip_mlp_set_precision(bacchus_hip_mlp_t *mlp, int32_t flags);
+
 const char *bacchus_hip_last_error(void);
 
 #ifdef __cplusplus

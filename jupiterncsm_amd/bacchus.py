@@ -54,6 +54,7 @@ EXPORTED_SYMBOLS = (
     "bacchus_hip_symmetric_eigen", "bacchus_hip_operator_load_blocks",
     "bacchus_hip_basis_project_static", "bacchus_hip_norm_matrix",
     "bacchus_hip_subspace_operator", "bacchus_hip_batch_mfma_stats",
+    "bacchus_hip_mlp_set_precision",
 )
 
 
@@ -661,12 +662,19 @@ def basis_transform(U, C):
     return out
 
 
+# bacchus_hip_mlp_set_precision flags
+MLP_SPLIT_ACTIVATIONS = 1
+MLP_FP32_LAST = 2
+
+
 class MLP:
     """Learned-start MLP on the device (BF16 MFMA forward/backward, FP32
     master weights, Adam).  Inputs and targets are float32 torch tensors on
     the GPU, row-major (M rows)."""
 
-    def __init__(self, inputs=2, hidden=256, depth=3, outputs=64, seed=1):
+    def __init__(self, inputs=2, hidden=256, depth=3, outputs=64, seed=1, precision=0):
+        """precision: MLP_SPLIT_ACTIVATIONS | MLP_FP32_LAST flags (0: every
+        GEMM operand rounded to BF16)."""
         lib = library()
         _bind_mlp(lib)
         h = ctypes.c_void_p()
@@ -674,6 +682,9 @@ class MLP:
         self._h = h
         self.shape = (inputs, hidden, depth, outputs)
         self.num_params = int(lib.bacchus_hip_mlp_num_params(h))
+        self.precision = int(precision)
+        if self.precision:
+            _check(lib.bacchus_hip_mlp_set_precision(h, self.precision))
 
     def close(self):
         if self._h is not None and self._h.value:
@@ -747,4 +758,6 @@ def _bind_mlp(lib):
     lib.bacchus_hip_mlp_train_step.restype = c_int
     lib.bacchus_hip_mlp_train_step.argtypes = [vp, vp, vp, c_int, ctypes.c_float,
                                                ctypes.POINTER(ctypes.c_float), vp]
+    lib.bacchus_hip_mlp_set_precision.restype = c_int
+    lib.bacchus_hip_mlp_set_precision.argtypes = [vp, ctypes.c_int32]
     _MLP_BOUND = True

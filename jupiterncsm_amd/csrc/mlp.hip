@@ -57,10 +57,13 @@ __device__ __forceinline__ float gelu_grad(float z)
 // v_mfma_f32_16x16x16_bf16 and FP32 accumulation; one wave per 16 x 16
 // tile.  Lane l holds A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and
 // D[4(l/16)+j][l%16], j = 0..3.  mode: 0 store, 1 GELU (Z to z_out, GELU(Z)
-// to C), 2 accumulate into C.
+// to C), 2 accumulate into C.  split: an operand with its bit set (1 = A,
+// 2 = B) is carried as two BF16 parts v = hi + lo, hi = bf16(v), lo =
+// bf16(v - hi) (16 significant bits), and the product takes the extra
+// MFMAs lo_A hi_B and hi_A lo_B.
 __global__ __launch_bounds__(64) void gemm_bf16(Operand A, Operand B, int M, int N, int K,
 						const float *bias, float *C, int64_t ldc,
-						float *z_out, int mode)
+						float *z_out, int mode, int split)
 {
 	const int tn = (N + 15) / 16;
 	const int tile = blockIdx.x;
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(64) void gemm_bf16(Operand A, Operand B, int M, int
 	const int row = l % 16, kg = l / 16;
 	f32x4 acc = {0, 0, 0, 0};
 	for (int k0 = 0; k0 < K; k0 += 16) {
-		bf16x4 a, b;
+		bf16x4 a, b, alo, blo;
 		for (int j = 0; j < 4; j++) {
 			const int k = k0 + 4 * kg + j;
 			const int am = m0 + row, bn = n0 + row;
@@ -77,8 +80,14 @@ __global__ __launch_bounds__(64) void gemm_bf16(Operand A, Operand B, int M, int
 			const float bv = (bn < N && k < K) ? load_op(B, k, bn) : 0.0f;
 			a[j] = to_bf16_bits(av);
 			b[j] = to_bf16_bits(bv);
+			alo[j] = to_bf16_bits(av - bf16_to_float((uint16_t)a[j]));
+			blo[j] = to_bf16_bits(bv - bf16_to_float((uint16_t)b[j]));
 		}
 		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+		if (split & 1)
+			acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, b, acc, 0, 0, 0);
+		if (split & 2)
+			acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, blo, acc, 0, 0, 0);
 	}
 	for (int j = 0; j < 4; j++) {
 		const int m = m0 + 4 * kg + j, n = n0 + row;
@@ -179,6 +188,7 @@ struct bacchus_hip_mlp {
 	uint16_t *params_bf16 = nullptr;       // BF16 copy (weights used)
 	float *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
 	int64_t step = 0;
+	int precision = 0;                     // BACCHUS_HIP_MLP_* flags
 	// activations for a batch of up to cap rows
 	int cap = 0;
 	std::vector<float *> Z, H;             // per layer pre-/post-activation
@@ -222,23 +232,40 @@ int ensure_capacity(bacchus_hip_mlp *m, int M)
 }
 
 void gemm(hipStream_t s, Operand A, Operand B, int M, int N, int K, const float *bias,
-	  float *C, int64_t ldc, float *z, int mode)
+	  float *C, int64_t ldc, float *z, int mode, int split)
 {
 	const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
 	hipLaunchKernelGGL(gemm_bf16, dim3(tiles), dim3(64), 0, s, A, B, M, N, K, bias, C, ldc,
-			   z, mode);
+			   z, mode, split);
+}
+
+// The weights of layer l as the GEMM operand W[n, k] at (k, n) (strides
+// as given): the BF16 copy, or for the last layer under
+// BACCHUS_HIP_MLP_FP32_LAST the FP32 master weights, carried as two BF16
+// parts (split flag 2 of gemm_bf16).
+Operand weights(const bacchus_hip_mlp *m, int l, int64_t rs, int64_t cs, int *split_b)
+{
+	if (l + 1 == m->nlayers && (m->precision & BACCHUS_HIP_MLP_FP32_LAST)) {
+		*split_b = 2;
+		return Operand{m->params + m->w_off[l], rs, cs, 0};
+	}
+	*split_b = 0;
+	return Operand{m->params_bf16 + m->w_off[l], rs, cs, 1};
 }
 
 // forward pass; the last layer is linear
 void forward(bacchus_hip_mlp *m, const float *x, int M, hipStream_t s)
 {
+	const int sa = (m->precision & BACCHUS_HIP_MLP_SPLIT_ACTIVATIONS) ? 1 : 0;
 	for (int l = 0; l < m->nlayers; l++) {
 		const int K = m->din[l], N = m->dout[l];
 		Operand A = l == 0 ? Operand{x, K, 1, 0} : Operand{m->H[l - 1], K, 1, 0};
-		// B(k, n) = W[n, k] (W is N x K row-major), BF16 copy
-		Operand B{m->params_bf16 + m->w_off[l], 1, K, 1};
+		// B(k, n) = W[n, k] (W is N x K row-major)
+		int sb;
+		Operand B = weights(m, l, 1, K, &sb);
 		const bool last = l + 1 == m->nlayers;
-		gemm(s, A, B, M, N, K, m->params + m->b_off[l], m->H[l], N, m->Z[l], last ? 0 : 1);
+		gemm(s, A, B, M, N, K, m->params + m->b_off[l], m->H[l], N, m->Z[l], last ? 0 : 1,
+		     sa | sb);
 	}
 }
 
@@ -249,19 +276,23 @@ void backward(bacchus_hip_mlp *m, const float *x, const float *t, int M, hipStre
 	const int64_t ny = (int64_t)M * m->out;
 	hipLaunchKernelGGL(mse_grad, dim3(1), dim3(256), 0, s, m->H[L - 1], t, ny, m->dY, m->loss);
 	const float *dz = m->dY;
+	// with split activations every backward operand but the weights is
+	// carried as two BF16 parts too
+	const int sa = (m->precision & BACCHUS_HIP_MLP_SPLIT_ACTIVATIONS) ? 1 : 0;
 	for (int l = L - 1; l >= 0; l--) {
 		const int K = m->din[l], N = m->dout[l];
 		const float *a = l == 0 ? x : m->H[l - 1];
 		// dW[n, k] = sum_m dZ[m, n] a[m, k]
 		gemm(s, Operand{dz, 1, N, 0}, Operand{a, K, 1, 0}, N, K, M, nullptr,
-		     m->grads + m->w_off[l], K, nullptr, 0);
+		     m->grads + m->w_off[l], K, nullptr, 0, sa ? 3 : 0);
 		hipLaunchKernelGGL(column_sum, dim3(grid(N)), dim3(256), 0, s, dz, M, N,
 				   m->grads + m->b_off[l]);
 		if (l == 0)
 			break;
 		// dA[m, k] = sum_n dZ[m, n] W[n, k]
-		gemm(s, Operand{dz, N, 1, 0}, Operand{m->params_bf16 + m->w_off[l], K, 1, 1}, M, K,
-		     N, nullptr, m->dA, K, nullptr, 0);
+		int sb;
+		Operand W = weights(m, l, K, 1, &sb);
+		gemm(s, Operand{dz, N, 1, 0}, W, M, K, N, nullptr, m->dA, K, nullptr, 0, sa | sb);
 		hipLaunchKernelGGL(gelu_backward, dim3(grid((int64_t)M * K)), dim3(256), 0, s, m->dA,
 				   m->Z[l - 1], m->dZ, (int64_t)M * K);
 		dz = m->dZ;
@@ -342,6 +373,16 @@ void bacchus_hip_mlp_free(bacchus_hip_mlp_t *m)
 }
 
 int64_t bacchus_hip_mlp_num_params(const bacchus_hip_mlp_t *m) { return m ? m->nparams : -1; }
+
+int bacchus_hip_mlp_set_precision(bacchus_hip_mlp_t *m, int32_t flags)
+{
+	if (!m)
+		return BACCHUS_FAIL("NULL argument");
+	if (flags & ~(BACCHUS_HIP_MLP_SPLIT_ACTIVATIONS | BACCHUS_HIP_MLP_FP32_LAST))
+		return BACCHUS_FAIL("unknown MLP precision flags %d", (int)flags);
+	m->precision = flags;
+	return 0;
+}
 
 int bacchus_hip_mlp_get_params(const bacchus_hip_mlp_t *m, float *host)
 {

@@ -181,7 +181,7 @@ def orthonormal_basis(U, threshold=1e-12):
 
 
 def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
-        seed=1, target="ground", log=None):
+        seed=1, target="ground", log=None, precision=0):
     """Stored basis: the orthonormalized span of the first basis_size
     targets (a fixed set of basis_size FP32 vectors, zero rows past the
     numerical rank).  The MLP is trained on all points to the coordinates
@@ -198,14 +198,15 @@ def fit(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000,
     if log:
         log(f"basis rank {rank} of {n}")
     mlp, mean, scale, loss = _train_coefficients(U, T, solves.points, hidden, depth, steps, lr,
-                                                 seed, log)
+                                                 seed, log, precision)
     return LearnedStart(U, mlp, mean, scale, loss, target, rank)
 
 
-def _train_coefficients(U, T, points, hidden, depth, steps, lr, seed, log):
+def _train_coefficients(U, T, points, hidden, depth, steps, lr, seed, log, precision=0):
     """An MLP (c1, c2) -> coordinates of the targets T (P, D) over the
     stored basis U, trained on the device; returns (mlp, mean, scale,
-    final loss) with the outputs standardized as (Z - mean) / scale."""
+    final loss) with the outputs standardized as (Z - mean) / scale.
+    precision: bacchus.MLP precision flags of the network's GEMMs."""
     import torch
     n = U.shape[0]
     Z = []
@@ -218,7 +219,7 @@ def _train_coefficients(U, T, points, hidden, depth, steps, lr, seed, log):
     X = (np.asarray(points) - 1.0) / 0.2
     xd = torch.as_tensor(X, dtype=torch.float32, device="cuda").contiguous()
     td = torch.as_tensor((Z - mean) / scale, dtype=torch.float32, device="cuda").contiguous()
-    mlp = bacchus.MLP(2, hidden, depth, n, seed=seed)
+    mlp = bacchus.MLP(2, hidden, depth, n, seed=seed, precision=precision)
     loss = None
     for k in range(steps):
         want = (k + 1) % 500 == 0 or k + 1 == steps
@@ -266,7 +267,7 @@ class LearnedBlockStart:
 
 
 def fit_block(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps=3000, lr=2e-3,
-              seed=1, log=None):
+              seed=1, log=None, precision=0):
     """Stored basis as in fit(); one MLP per band_targets entry of the
     solves, each of the shape fit() trains."""
     import torch
@@ -277,7 +278,7 @@ def fit_block(solves: FamilySolves, *, basis_size=64, hidden=256, depth=3, steps
     models, losses = [], []
     for k, T in enumerate(solves.band_targets):
         mlp, mean, scale, loss = _train_coefficients(U, T, solves.points, hidden, depth, steps,
-                                                     lr, seed + 101 * k, log)
+                                                     lr, seed + 101 * k, log, precision)
         models.append((mlp, mean, scale))
         losses.append(loss)
     torch.cuda.synchronize()

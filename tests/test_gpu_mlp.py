@@ -59,3 +59,45 @@ def test_training_reduces_loss():
     last = mlp.train_step(xd, td, lr=3e-3, want_loss=True)
     assert last < 0.05 * first
     mlp.close()
+
+
+@pytest.mark.parametrize("precision", [bacchus.MLP_SPLIT_ACTIVATIONS,
+                                       bacchus.MLP_SPLIT_ACTIVATIONS | bacchus.MLP_FP32_LAST])
+def test_split_precision_matches_oracle(precision):
+    """Activations (and the output layer's weights) carried as two BF16
+    parts: forward and gradients against the oracle emulating the same
+    parts; with weights that are BF16 values already, the forward is then
+    the FP64 network to FP32-level accuracy, where plain BF16 operands are
+    off by ~1e-3."""
+    import torch
+    print(f"case mlp split precision {precision}", flush=True)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1.0, 1.0, (37, 2)).astype(np.float32)
+    t = (rng.standard_normal((37, 64)) * 0.1).astype(np.float32)
+    xd, td = torch.as_tensor(x, device="cuda"), torch.as_tensor(t, device="cuda")
+    mlp = bacchus.MLP(2, 256, 3, 64, seed=6, precision=precision)
+    p = mlp.params().astype(np.float64)
+    y = mlp.forward(xd).cpu().numpy()
+    y_ref, *_ = M.forward(p, mlp.shape, x, bf16=True, precision=precision)
+    assert np.abs(y - y_ref).max() <= 1e-4 * np.abs(y_ref).max()
+    loss, g = mlp.gradients(xd, td)
+    loss_ref, g_ref = M.loss_and_gradients(p, mlp.shape, x, t, bf16=True, precision=precision)
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    assert np.abs(g - g_ref).max() <= 2e-3 * np.abs(g_ref).max()
+    # weights on the BF16 grid: split activations leave only FP32 rounding
+    pb = M.bf16_round(p.astype(np.float32))
+    mlp.set_params(pb.astype(np.float32))
+    plain = bacchus.MLP(2, 256, 3, 64, seed=6)
+    plain.set_params(pb.astype(np.float32))
+    y_exact, *_ = M.forward(pb, mlp.shape, x)
+    e_split = np.abs(mlp.forward(xd).cpu().numpy() - y_exact).max() / np.abs(y_exact).max()
+    e_plain = np.abs(plain.forward(xd).cpu().numpy() - y_exact).max() / np.abs(y_exact).max()
+    print(f"vs FP64 network: split {e_split:.2e}, plain BF16 {e_plain:.2e}", flush=True)
+    assert e_split <= 1e-4 and e_split * 20 <= e_plain
+    mlp.close()
+    plain.close()
+
+
+def test_unknown_precision_flags_rejected():
+    with pytest.raises(bacchus.BacchusError, match="precision"):
+        bacchus.MLP(2, 8, 1, 2, precision=4)
