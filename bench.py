@@ -83,6 +83,8 @@ def parse():
                     help="further (c1, c2) points the fast path is checked on, each against "
                          "its own FP64 reference")
     ap.add_argument("--batched", action="store_true", help="batched 64-Hamiltonian sweep")
+    ap.add_argument("--batched-matvec-only", type=int, default=0,
+                    help="only time this many batched matvecs of the sweep's batch (profiling)")
     ap.add_argument("--learned", action="store_true", help="learned-start measurement")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -140,7 +142,7 @@ def main():
         torch.cuda.synchronize()
         log(f"copied {args.copy_calibration} MiB x 3")
         return
-    if args.batched:
+    if args.batched or args.batched_matvec_only:
         return batched_sweep(args)
     if args.learned:
         return learned_start(args)
@@ -626,6 +628,19 @@ def batched_sweep(args, emit=True):
     torch.cuda.synchronize()
     bmv = e0.elapsed_time(e1) / 20
     mfma = bt.mfma_stats()
+    if args.batched_matvec_only:
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.batched_matvec_only):
+            bt.matvec(xs, ys)
+        e1.record()
+        torch.cuda.synchronize()
+        log(f"batched matvec {e0.elapsed_time(e1) / args.batched_matvec_only:.4f} ms "
+            f"(P-form on the matrix cores: {mfma})")
+        for o in ops32:
+            o.close()
+        bt.close()
+        return None
     # the same batch with the P-form instructions on the vector path
     bt_v = batch_without_mfma(ops32, counts)
     for k, s in enumerate(split):
