@@ -147,8 +147,16 @@ def main():
     if args.learned:
         return learned_start(args)
     t0 = time.time()
-    inputs = build(SYSTEMS[args.workload])
-    log(f"inputs {args.workload}: {inputs.summary()} in {time.time() - t0:.1f}s")
+    inputs = build(SYSTEMS[args.workload], device=True)
+    gen_device = time.time() - t0
+    log(f"inputs {args.workload}: {inputs.summary()} in {gen_device:.2f}s (index lists and "
+        f"elements on the GPU)")
+    gen_host = None
+    if not args.matvec_only and not args.block_only:
+        t0 = time.time()
+        build(SYSTEMS[args.workload])
+        gen_host = time.time() - t0
+        log(f"the same inputs from the host generator: {gen_host:.2f}s")
     D = inputs.dimension
     op64 = bacchus.Operator.from_inputs(inputs, bacchus.FP64)
     stats = op64.stats()
@@ -476,6 +484,8 @@ def main():
         "amortized_seconds_per_solve": (
             {str(n): round(t_fast + train_s / n, 4) for n in (64, 512, 4096)}
             if fam is not None else None),
+        "input_generation_seconds": {"device": round(gen_device, 3),
+                                     "host": round(gen_host, 3) if gen_host else None},
         "matvec_fp64_ms": round(mv_ms, 4),
         "matvec_fp32_ms": round(mv32_ms, 4),
         "block_matvec": block_mv,

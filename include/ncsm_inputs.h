@@ -71,6 +71,15 @@ typedef struct ncsm_instruction {
 /* Build the basis, index lists, matrix-element blocks and instructions.
  * Returns NULL and prints a message on invalid configurations. */
 ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config);
+/* The same inputs with the two bulk stages on the GPU (current HIP device):
+ * the index-list entries (connection_list.c:377-446 contract; one device
+ * thread per input determinant and annihilated subset walks the creation
+ * candidates) and the matrix-element blocks (one thread per element, the
+ * same keyed Gaussians).  Index lists, instructions and shapes are
+ * identical to ncsm_inputs_build's; elements agree to a few units in the
+ * last place (the device's log and cos).  Returns NULL and prints a
+ * message on invalid configurations or device errors. */
+ncsm_inputs_t *ncsm_inputs_build_gpu(const ncsm_config_t *config);
 void ncsm_inputs_free(ncsm_inputs_t *inputs);
 
 /* Write comb.txt, evaluation_order, index_lists/ and matrix_elements/ under

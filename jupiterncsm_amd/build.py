@@ -43,9 +43,19 @@ def _run(cmd, verbose):
 
 
 def build_inputs(verbose=False):
+    """libncsm_inputs.so: the host generator (g++, OpenMP) and its device
+    stages (ncsm_inputs_gpu.hip, gfx950), linked by hipcc."""
     out = os.path.join(HERE, "libncsm_inputs.so")
-    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
-          f"-I{INCLUDE}", os.path.join(CSRC, "ncsm_inputs.cpp"), "-o", out], verbose)
+    bdir = os.path.join(CSRC, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    host_obj = os.path.join(bdir, "ncsm_inputs.o")
+    dev_obj = os.path.join(bdir, "ncsm_inputs_gpu.o")
+    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+          os.path.join(CSRC, "ncsm_inputs.cpp"), "-o", host_obj], verbose)
+    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+          os.path.join(CSRC, "ncsm_inputs_gpu.hip"), "-o", dev_obj], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", host_obj, dev_obj, "-lgomp",
+          "-o", out], verbose)
     return out
 
 

@@ -29,13 +29,18 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
+#include <sys/mman.h>
 #include <sys/stat.h>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
+
+#include "ncsm_gpu.h"
 
 namespace {
 
@@ -141,6 +146,24 @@ double keyed_gaussian(uint64_t seed, int rank, const int32_t *bra,
 
 }  // namespace
 
+namespace {
+// allocator whose resize() leaves new elements uninitialized: the element
+// array is written in full afterwards, so zeroing gigabytes first would
+// only add a pass
+template <typename T> struct NoInitAlloc : std::allocator<T> {
+	template <typename U> struct rebind {
+		using other = NoInitAlloc<U>;
+	};
+	NoInitAlloc() = default;
+	template <typename U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+	template <typename U> void construct(U *q) noexcept { ::new ((void *)q) U; }
+	template <typename U, typename... A> void construct(U *q, A &&...a)
+	{
+		::new ((void *)q) U(std::forward<A>(a)...);
+	}
+};
+}  // namespace
+
 struct ncsm_inputs {
 	ncsm_config_t cfg{};
 	std::vector<Orbital> orbs;
@@ -163,7 +186,7 @@ struct ncsm_inputs {
 	std::vector<VKey> vkeys;
 	std::vector<int64_t> v_offsets;
 	std::vector<int64_t> v_shapes;  // Dn, Dp, force
-	std::vector<double> v_elements;
+	std::vector<double, NoInitAlloc<double>> v_elements;
 
 	std::vector<ncsm_instruction_t> instr;
 	std::vector<int64_t> instr_mults;
@@ -492,7 +515,172 @@ const char *type_string(int kk, char c)
 
 extern "C" {
 
+namespace {
+
+// stage times on stderr when NCSM_VERBOSE is set
+struct StageClock {
+	bool on = getenv("NCSM_VERBOSE") != nullptr;
+	std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+	void mark(const char *what)
+	{
+		const auto now = std::chrono::steady_clock::now();
+		if (on)
+			fprintf(stderr, "[ncsm] %s %.3f s\n", what,
+				std::chrono::duration<double>(now - t).count());
+		t = now;
+	}
+};
+
+ncsm::Packed pack(const Tuple &t)
+{
+	ncsm::Packed p = 0;
+	for (int i = 0; i < kMaxParticles; i++)
+		p |= ncsm::Packed(uint16_t(t[i])) << (16 * i);
+	return p;
+}
+
+// Every list group's entries on the GPU (the work of build_list_group):
+// the host makes one item per (group, input determinant, annihilated
+// subset) in build_list_group's loop order, the device returns each item's
+// entries in candidate order, and they are appended to the item's depth
+// list, which gives the host generator's lists exactly.
+bool gpu_list_groups(ncsm_inputs &in, const std::vector<ListKey> &groups,
+		     std::vector<std::map<int, std::vector<int32_t>>> &group_lists)
+{
+	// state sets and sub-basis keys of both species, flattened
+	StageClock clk;
+	std::map<const StateSet *, int64_t> state_off;
+	std::vector<ncsm::Packed> states;
+	std::map<const SubBasis *, int64_t> key_off;
+	std::vector<uint64_t> keys;
+	for (int sp = 0; sp < 2; sp++) {
+		for (auto &kv : in.sp[sp].kstates) {
+			state_off[&kv.second] = (int64_t)states.size();
+			for (const Tuple &t : kv.second.states)
+				states.push_back(pack(t));
+		}
+		for (const SubBasis &sb : in.sp[sp].subs) {
+			key_off[&sb] = (int64_t)keys.size();
+			for (const Tuple &t : sb.dets)
+				keys.push_back(ncsm::key_be(t.data(), in.sp[sp].k));
+		}
+	}
+	std::vector<std::vector<ncsm::GenItem>> per(groups.size());
+	std::vector<std::vector<int>> depth(groups.size());
+#pragma omp parallel for schedule(dynamic, 1)
+	for (long gi = 0; gi < (long)groups.size(); gi++) {
+		const ListKey &key = groups[gi];
+		const Species &s = in.sp[key.species];
+		const SubBasis &sin = s.subs[key.sub_in];
+		const SubBasis &sout = s.subs[key.sub_out];
+		const int k = s.k, kk = key.k;
+		const int dE = sout.E - sin.E, dM = sout.m2 - sin.m2;
+		for (size_t din = 0; din < sin.dets.size(); din++) {
+			const Tuple &det = sin.dets[din];
+			// k'-subsets of the k positions in ascending order, as
+			// build_list_group enumerates them
+			std::array<int, kMaxParticles> idx{};
+			for (int i = 0; i < kk; i++)
+				idx[i] = i;
+			while (true) {
+				Tuple C{};
+				C.fill(-1);
+				uint8_t mask = 0;
+				for (int c = 0; c < kk; c++) {
+					C[c] = det[idx[c]];
+					mask |= uint8_t(1u << idx[c]);
+				}
+				const int d = tuple_energy(in.orbs, C, kk);
+				const int mC = tuple_m2(in.orbs, C, kk);
+				const StateSet *ann = find_states(s, kk, d);
+				const StateSet *cre = find_states(s, kk, d + dE);
+				if (ann && cre) {
+					auto g = cre->mgroup.find(mC + dM);
+					if (g != cre->mgroup.end()) {
+						const auto ct = in.conn_tables.find(
+							std::make_tuple(key.species, kk, d, dE, dM));
+						const int ann_pos = ann->index.at(tuple_key(C, kk));
+						ncsm::GenItem it{};
+						it.det = pack(det);
+						it.cre_first = state_off.at(cre) + g->second.first;
+						it.m_base = ct->second.offset[ann_pos];
+						it.out_first = key_off.at(&sout);
+						it.out_count = (int32_t)sout.dets.size();
+						it.cre_count = g->second.second;
+						it.din = (int32_t)din;
+						it.sub = mask;
+						it.k = (uint8_t)k;
+						it.kk = (uint8_t)kk;
+						per[gi].push_back(it);
+						depth[gi].push_back(d);
+					}
+				}
+				int i = kk - 1;
+				while (i >= 0 && idx[i] == k - kk + i)
+					i--;
+				if (i < 0)
+					break;
+				idx[i]++;
+				for (int j = i + 1; j < kk; j++)
+					idx[j] = idx[j - 1] + 1;
+			}
+		}
+	}
+	std::vector<ncsm::GenItem> items;
+	for (auto &v : per)
+		items.insert(items.end(), v.begin(), v.end());
+	clk.mark("list work items (host)");
+	std::vector<int32_t> entries, counts;
+	std::string err;
+	const int rc = ncsm::gpu_list_entries(items, states, keys, entries, counts, err);
+	clk.mark("list entries (device)");
+	if (rc) {
+		fprintf(stderr, "ncsm_inputs_build_gpu: index lists: %s\n", err.c_str());
+		return false;
+	}
+	// every group's first item and first entry, then the groups split in
+	// parallel
+	std::vector<size_t> item0(groups.size() + 1, 0), entry0(groups.size() + 1, 0);
+	for (size_t gi = 0; gi < groups.size(); gi++) {
+		item0[gi + 1] = item0[gi] + per[gi].size();
+		size_t n = 0;
+		for (size_t j = item0[gi]; j < item0[gi + 1]; j++)
+			n += counts[j];
+		entry0[gi + 1] = entry0[gi] + n;
+	}
+#pragma omp parallel for schedule(dynamic, 1)
+	for (long gi = 0; gi < (long)groups.size(); gi++) {
+		size_t e = entry0[gi];
+		for (size_t j = 0; j < per[gi].size(); j++) {
+			const int32_t n = counts[item0[gi] + j];
+			if (!n)
+				continue;
+			std::vector<int32_t> &v = group_lists[gi][depth[gi][j]];
+			v.insert(v.end(), entries.begin() + 3 * e, entries.begin() + 3 * (e + n));
+			e += n;
+		}
+	}
+	clk.mark("list split by depth (host)");
+	return true;
+}
+
+ncsm_inputs_t *build_inputs(const ncsm_config_t *config, bool gpu);
+
+}  // namespace
+
 ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
+{
+	return build_inputs(config, false);
+}
+
+ncsm_inputs_t *ncsm_inputs_build_gpu(const ncsm_config_t *config)
+{
+	return build_inputs(config, true);
+}
+
+namespace {
+
+ncsm_inputs_t *build_inputs(const ncsm_config_t *config, bool gpu)
 {
 	if (!config)
 		return nullptr;
@@ -505,6 +693,7 @@ ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
 			c.num_protons, c.num_neutrons, c.nmax, c.max_body);
 		return nullptr;
 	}
+	StageClock clk;
 	ncsm_inputs *in = new ncsm_inputs;
 	in->cfg = c;
 	// single-particle basis large enough for any orbital the Nmax
@@ -639,13 +828,22 @@ ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
 				conn_table(*in, g.species, g.k, kv.first.second,
 					   dE, dM);
 	}
+	clk.mark("bases, blocks, list groups, connection tables");
 	// pass 2: build every list group
 	std::vector<std::map<int, std::vector<int32_t>>> group_lists(
 		groups.size());
+	if (gpu) {
+		if (!gpu_list_groups(*in, groups, group_lists)) {
+			delete in;
+			return nullptr;
+		}
+	} else {
 #pragma omp parallel for schedule(dynamic, 1)
-	for (long gi = 0; gi < (long)groups.size(); gi++)
-		group_lists[gi] = build_list_group(*in, groups[gi]);
+		for (long gi = 0; gi < (long)groups.size(); gi++)
+			group_lists[gi] = build_list_group(*in, groups[gi]);
+	}
 
+	clk.mark("index lists");
 	// pass 3: ids in order of first use, instructions
 	std::map<std::pair<int, int>, int> list_id;  // (group, depth) -> id
 	std::map<VKey, int> v_id;
@@ -753,6 +951,7 @@ ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
 	for (ListData &ld : in->lists)
 		std::vector<int32_t>().swap(ld.entries);
 
+	clk.mark("instructions, flattened lists");
 	// pass 4: matrix-element blocks
 	const size_t nv = in->vkeys.size();
 	std::vector<ConnList> cp(nv), cn(nv);
@@ -767,7 +966,64 @@ ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
 		in->v_offsets.push_back(in->v_offsets.back() + n);
 		in->v_shapes.insert(in->v_shapes.end(), {Dn, Dp, k.kp + k.kn});
 	}
-	in->v_elements.assign(in->v_offsets.back(), 0.0);
+	clk.mark("connection pairs of the element blocks");
+	in->v_elements.resize(in->v_offsets.back());
+	{
+		// transparent huge pages for the element array: its first
+		// touch (the writes below) then faults in 2 MiB pages, not 4 KiB
+		const uintptr_t a = ((uintptr_t)in->v_elements.data() + 4095) & ~(uintptr_t)4095;
+		const uintptr_t b = (uintptr_t)(in->v_elements.data() + in->v_elements.size());
+		if (b > a + (2u << 20))
+			(void)madvise((void *)a, b - a, MADV_HUGEPAGE);
+	}
+	clk.mark("element array");
+	if (gpu) {
+		// one device thread per element: the pairs of every block's
+		// proton and neutron parts, packed
+		std::vector<ncsm::ElemBlock> eb(nv);
+		int64_t npairs = 0;
+		for (size_t v = 0; v < nv; v++) {
+			const VKey &k = in->vkeys[v];
+			ncsm::ElemBlock &b = eb[v];
+			b.offset = in->v_offsets[v];
+			b.pair_first = npairs;
+			b.np = (int32_t)cp[v].pairs.size();
+			b.nn = (int32_t)cn[v].pairs.size();
+			b.kp = k.kp;
+			b.kn = k.kn;
+			b.sigma = k.kp + k.kn == 2 ? c.sigma2 : c.sigma3;
+			npairs += b.np + b.nn;
+		}
+		std::vector<ncsm::Packed> pairs(2 * npairs);
+#pragma omp parallel for schedule(dynamic, 16)
+		for (long v = 0; v < (long)nv; v++) {
+			const VKey &k = in->vkeys[v];
+			ncsm::Packed *dst = pairs.data() + 2 * eb[v].pair_first;
+			for (int part = 0; part < 2; part++) {
+				const ConnList &L = part == 0 ? cp[v] : cn[v];
+				const int kk = part == 0 ? k.kp : k.kn;
+				for (const auto &pr : L.pairs) {
+					*dst++ = kk ? pack(L.ann->states[pr.first]) : 0;
+					*dst++ = kk ? pack(L.cre->states[pr.second]) : 0;
+				}
+			}
+		}
+		std::vector<int32_t> orbital_e;
+		for (const Orbital &o : in->orbs)
+			orbital_e.push_back(o.e);
+		clk.mark("element pairs (host)");
+		std::string err;
+		const int rc = ncsm::gpu_elements(eb, pairs, orbital_e, c.seed, c.hw,
+						  in->v_elements.data(),
+						  (int64_t)in->v_elements.size(), err);
+		clk.mark("elements (device)");
+		if (rc) {
+			fprintf(stderr, "ncsm_inputs_build_gpu: matrix elements: %s\n", err.c_str());
+			delete in;
+			return nullptr;
+		}
+		return in;
+	}
 	for (size_t v = 0; v < nv; v++) {
 		const VKey &k = in->vkeys[v];
 		const int rank = k.kp + k.kn;
@@ -799,8 +1055,11 @@ ncsm_inputs_t *ncsm_inputs_build(const ncsm_config_t *config)
 			}
 		}
 	}
+	clk.mark("elements (host)");
 	return in;
 }
+
+}  // namespace
 
 void ncsm_inputs_free(ncsm_inputs_t *inputs) { delete inputs; }
 

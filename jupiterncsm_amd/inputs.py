@@ -51,10 +51,16 @@ def library() -> ctypes.CDLL:
     path = os.path.join(_HERE, "libncsm_inputs.so")
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+    # the library links the HIP runtime for its device stages; torch bundles
+    # its own under the same SONAME, and loading torch first binds this
+    # library to that one (one runtime per process)
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
     lib.ncsm_inputs_build.restype = vp
     lib.ncsm_inputs_build.argtypes = [ctypes.POINTER(_Config)]
+    lib.ncsm_inputs_build_gpu.restype = vp
+    lib.ncsm_inputs_build_gpu.argtypes = [ctypes.POINTER(_Config)]
     lib.ncsm_inputs_free.argtypes = [vp]
     lib.ncsm_inputs_write.restype = ctypes.c_int
     lib.ncsm_inputs_write.argtypes = [vp, ctypes.c_char_p]
@@ -112,14 +118,16 @@ class Inputs:
     """Owner of one generated system; arrays are read-only numpy views into
     the native object and live as long as this instance."""
 
-    def __init__(self, config: SystemConfig):
+    def __init__(self, config: SystemConfig, device: bool = False):
         lib = library()
         c = _Config(config.num_protons, config.num_neutrons, config.nmax,
                     config.twice_m, config.max_body, int(config.include_one_body),
                     config.seed, config.hw, config.sigma2, config.sigma3)
-        handle = lib.ncsm_inputs_build(ctypes.byref(c))
+        build_fn = lib.ncsm_inputs_build_gpu if device else lib.ncsm_inputs_build
+        handle = build_fn(ctypes.byref(c))
         if not handle:
-            raise ValueError(f"invalid system configuration {config}")
+            raise ValueError(f"invalid system configuration {config}"
+                             + (" or device failure" if device else ""))
         self.config = config
         self._lib = lib
         self._h = ctypes.c_void_p(handle)
@@ -216,8 +224,10 @@ class Inputs:
         }
 
 
-def build(config: SystemConfig) -> Inputs:
-    return Inputs(config)
+def build(config: SystemConfig, device: bool = False) -> Inputs:
+    """The system's inputs; device=True runs the index-list and
+    matrix-element stages on the GPU (ncsm_inputs_build_gpu)."""
+    return Inputs(config, device)
 
 
 # The workloads the benchmarks and tests use (sizes are the generator's).
