@@ -249,6 +249,8 @@ typedef struct bacchus_hip_batch_tuning {
 	int32_t max_tile_vectors; /* vectors per tile, power of two <= 64; 0 = 64 */
 	int32_t workgroups_per_cu; /* persistent grid; 0 = 8                    */
 	int32_t tile_words;       /* FP64 LDS sums per tile, <= 8192; 0 = 4096   */
+	int32_t waves_per_simd;   /* 4: kernel held to 128 VGPRs (four waves per
+				     SIMD); 0 = the compiler's register count */
 } bacchus_hip_batch_tuning_t;
 
 int bacchus_hip_batch_create(bacchus_hip_operator_t *const *ops,
@@ -266,8 +268,9 @@ int bacchus_hip_batch_set_scales(bacchus_hip_batch_t *batch, int32_t op_index,
 int bacchus_hip_batch_matvec(bacchus_hip_batch_t *batch, const float *const *x,
 			     float *const *y, void *stream);
 
-/* Times candidate tilings (repetitions matvecs each, on internal buffers)
- * and keeps the fastest; writes it and its time per batched matvec. */
+/* Times candidate tilings (repetitions matvecs each, on internal buffers),
+ * then the best of them with the other waves_per_simd, and keeps the
+ * fastest; writes it and its time per batched matvec. */
 int bacchus_hip_batch_autotune(bacchus_hip_batch_t *batch, int32_t repetitions,
 			       bacchus_hip_batch_tuning_t *best, double *best_ms);
 

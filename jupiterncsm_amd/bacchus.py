@@ -88,7 +88,8 @@ class _LanczosSettings(ctypes.Structure):
 
 class _BatchTuning(ctypes.Structure):
     _fields_ = [("tiles_per_cu", ctypes.c_int32), ("max_tile_vectors", ctypes.c_int32),
-                ("workgroups_per_cu", ctypes.c_int32), ("tile_words", ctypes.c_int32)]
+                ("workgroups_per_cu", ctypes.c_int32), ("tile_words", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32)]
 
 
 class _BatchLanczosSettings(ctypes.Structure):
@@ -422,7 +423,7 @@ class Batch:
     launch.  Vectors are float32 device tensors of shape (D_i, B_i)."""
 
     def __init__(self, operators, num_vectors, tiles_per_cu=0, max_tile_vectors=0,
-                 workgroups_per_cu=0, tile_words=0):
+                 workgroups_per_cu=0, tile_words=0, waves_per_simd=0):
         lib = library()
         self.operators = list(operators)
         self.num_vectors = [int(b) for b in num_vectors]
@@ -430,7 +431,8 @@ class Batch:
             raise BacchusError("one vector count per operator")
         hs = (ctypes.c_void_p * len(self.operators))(*[o.handle.value for o in self.operators])
         nv = np.ascontiguousarray(self.num_vectors, dtype=np.int32)
-        tu = _BatchTuning(tiles_per_cu, max_tile_vectors, workgroups_per_cu, tile_words)
+        tu = _BatchTuning(tiles_per_cu, max_tile_vectors, workgroups_per_cu, tile_words,
+                          waves_per_simd)
         h = ctypes.c_void_p()
         _check(lib.bacchus_hip_batch_create(ctypes.cast(hs, ctypes.c_void_p), _ptr(nv),
                                             len(self.operators), ctypes.byref(tu), ctypes.byref(h)))
@@ -471,6 +473,7 @@ class Batch:
                                                     ctypes.byref(best), ctypes.byref(ms)))
         return {"tiles_per_cu": best.tiles_per_cu, "max_tile_vectors": best.max_tile_vectors,
                 "workgroups_per_cu": best.workgroups_per_cu, "tile_words": best.tile_words,
+                "waves_per_simd": best.waves_per_simd,
                 "ms": ms.value}
 
     def stats(self) -> dict:

@@ -175,6 +175,41 @@ template <typename S> __device__ __forceinline__ S unpack_words(uint32_t w)
 	return s;
 }
 
+// Loads at a uniform base plus a 32-bit per-lane byte offset, as raw
+// buffer loads: the base and the range (bytes) form a buffer resource in
+// SGPRs, the lane needs one VGPR of offset per load, and an offset beyond
+// the range reads 0
+__device__ __forceinline__ const char *uniform_ptr(const char *p)
+{
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (const char *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const char *base, uint32_t bytes)
+{
+	// third dword: the raw-buffer format bits of gfx9 targets
+	return __builtin_amdgcn_make_buffer_rsrc((void *)uniform_ptr(base), (short)0,
+						 (int)__builtin_amdgcn_readfirstlane(bytes),
+						 0x00020000);
+}
+__device__ __forceinline__ float4 ld4_at(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+	const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+	return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+			   __uint_as_float(v[3]));
+}
+template <typename VT> __device__ __forceinline__ float elem_at(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float elem_at<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ float elem_at<uint16_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	return __uint_as_float(uint32_t(__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)) << 16);
+}
+
 template <typename VT> struct Load;
 template <> struct Load<float> {
 	__device__ static float at(const float *p, size_t i) { return global_of(p)[i]; }
@@ -263,9 +298,12 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				// loads of empty chunks are skipped
 				const int nc = (int)min((uint32_t)kLaneEntries, (T.e1 - e0 + G - 1) / (uint32_t)G);
 				bool vl[kLaneEntries];
-				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
+				// byte offsets of the lane's element (from the element
+				// row) and of its 4 input values (from the input row)
+				uint32_t colb[kLaneEntries], xob[kLaneEntries], outc[kLaneEntries];
 				float sgn[kLaneEntries];
 				float4 acc[kLaneEntries];
+				const uint32_t bl = bvalid ? (uint32_t)b : 0u;
 #pragma unroll
 				for (int c = 0; c < kLaneEntries; c++) {
 					const uint32_t e = e0 + (uint32_t)(c * G + slot);
@@ -274,8 +312,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					const uint2 en = global_of(o.ent)[k];
 					// element column: compact index, or lane-list
 					// position for expanded element rows
-					col[c] = I.expanded ? k - I.n_base : en.y & kMask;
-					xin[c] = en.x;
+					colb[c] = (I.expanded ? k - I.n_base : en.y & kMask) * (uint32_t)sizeof(VT);
+					xob[c] = (en.x * (uint32_t)B + bl) * 4u;
 					outc[c] = global_of(o.keys)[k];
 					sgn[c] = sign_of(1.f, en.y);
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -290,13 +328,21 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					constexpr int k = decltype(buf)::value;
 					const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 					const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-					const size_t vrow = (size_t)dn * (py & kMask);
-					const size_t xrow = (size_t)nn_in * px;
+					// element row and input row: uniform bases
+					// a lane's column (compact index or lane-list
+					// position) is below the lane list's length
+					const __amdgpu_buffer_rsrc_t vr = rsrc_of(
+						reinterpret_cast<const char *>(vblk + (size_t)dn * (py & kMask)),
+						(uint32_t)I.n_len * (uint32_t)sizeof(VT));
+					const __amdgpu_buffer_rsrc_t xr = rsrc_of(
+						reinterpret_cast<const char *>(
+							xs + ((size_t)I.x_off + (size_t)nn_in * px) * B),
+						nn_in * (uint32_t)B * 4u);
 #pragma unroll
 					for (int c = 0; c < kLaneEntries; c++) {
 						if (kFull || c < nc) {
-							va[k][c] = Load<VT>::at(vblk, vrow + col[c]);
-							xa[k][c] = ld4(xb + (xrow + xin[c]) * B);
+							va[k][c] = elem_at<VT>(vr, colb[c]);
+							xa[k][c] = ld4_at(xr, xob[c]);
 						} else {
 							va[k][c] = 0.f;
 							xa[k][c] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -462,9 +508,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 // XCD (workgroup i runs on XCD i mod kXcds) until it is exhausted, then
 // from the others in turn.  The queue heads next[q] are cleared by
 // reset_queue before every launch.
-__global__ __launch_bounds__(kTileThreads) void batch_kernel(
-	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
-	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
+__device__ __forceinline__ void batch_body(const BatchOp *__restrict__ ops,
+					   const BatchItem *__restrict__ items,
+					   const int *__restrict__ qbegin, int nq, int *__restrict__ next)
 {
 	extern __shared__ double tile[];
 	__shared__ int s_item;
@@ -490,6 +536,22 @@ __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 			__syncthreads();
 		}
 	}
+}
+
+// The same kernel compiled twice: with the registers the compiler chooses
+// (130 VGPRs, three waves per SIMD) and held to 128 VGPRs for a fourth
+// wave; the autotuner keeps the faster for the batch.
+__global__ __launch_bounds__(kTileThreads) void batch_kernel(
+	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
+	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
+{
+	batch_body(ops, items, qbegin, nq, next);
+}
+__global__ __launch_bounds__(kTileThreads, 4) void batch_kernel_w4(
+	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
+	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
+{
+	batch_body(ops, items, qbegin, nq, next);
 }
 
 typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
@@ -1115,7 +1177,8 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 	}
 	if (bt->num_items) {
 		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next, bt->nq);
-		hipLaunchKernelGGL(batch_kernel, dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
+		hipLaunchKernelGGL(bt->tuning.waves_per_simd == 4 ? batch_kernel_w4 : batch_kernel,
+				   dim3(bt->grid), dim3(kTileThreads), bt->lds_bytes, s,
 				   bt->d_ops, bt->d_items, bt->d_qbegin, bt->nq, bt->d_next);
 	}
 	if (bt->mfma_items)
@@ -1411,7 +1474,7 @@ int bacchus_hip_batch_autotune(bacchus_hip_batch_t *bt, int32_t reps,
 		for (int v : vec)
 			for (int g : wpc)
 				for (int wd : wds) {
-					bacchus_hip_batch_tuning_t t{a, v, g, wd};
+					bacchus_hip_batch_tuning_t t{a, v, g, wd, bt->tuning.waves_per_simd};
 					bt->tuning = t;
 					if ((rc = build_plan(bt)))
 						goto done;
@@ -1433,6 +1496,31 @@ int bacchus_hip_batch_autotune(bacchus_hip_batch_t *bt, int32_t reps,
 						best_t = t;
 					}
 				}
+	// the best tiling again with the kernel held to four waves per SIMD
+	{
+		bacchus_hip_batch_tuning_t t = best_t;
+		t.waves_per_simd = best_t.waves_per_simd == 4 ? 0 : 4;
+		bt->tuning = t;
+		if ((rc = build_plan(bt)))
+			goto done;
+		if ((rc = launch_batch(bt, nullptr)))
+			goto done;
+		(void)hipEventRecord(e0, nullptr);
+		for (int r = 0; r < reps; r++)
+			if ((rc = launch_batch(bt, nullptr)))
+				goto done;
+		(void)hipEventRecord(e1, nullptr);
+		if (hipEventSynchronize(e1) != hipSuccess) {
+			rc = BACCHUS_FAIL("autotune run failed");
+			goto done;
+		}
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		if (ms / reps < best_v) {
+			best_v = ms / reps;
+			best_t = t;
+		}
+	}
 	bt->tuning = best_t;
 	rc = build_plan(bt);
 done:

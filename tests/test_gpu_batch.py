@@ -75,6 +75,17 @@ def test_batched_matvec_vector_widths(counts, vectors):
     assert max(errs) <= 2e-6
 
 
+@pytest.mark.parametrize("precision,tol", [(bacchus.FP32, 2e-6), (bacchus.BF16, 2e-2)])
+def test_batched_matvec_four_waves_per_simd(precision, tol):
+    """The kernel variant held to 128 VGPRs (waves_per_simd = 4, one of the
+    autotuner's choices) on the uneven three-operator batch."""
+    print(f"case batch four waves {precision}", flush=True)
+    errs = batch_vs_oracle([SYSTEMS["he4_nmax4"], SYSTEMS["he4_nmax6"], SMALL_3B], [3, 5, 2],
+                           precision, seed=9, waves_per_simd=4)
+    print("errors", errs, flush=True)
+    assert max(errs) <= tol
+
+
 def test_batched_matvec_c2_size():
     """2+2 Nmax 8 at 16 vectors against the FP64 single-vector GPU path
     (pinned to the oracle elsewhere): linear in the scales."""
