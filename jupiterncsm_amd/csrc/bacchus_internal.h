@@ -291,6 +291,54 @@ int np_launch(const bacchus_hip_operator *op, const void *x, const void *xT, dou
 // Frees the NP tables.
 void np_free(bacchus_hip_operator *op);
 
+// Loads at a uniform base plus a 32-bit per-lane byte offset, as raw
+// buffer loads: the base and the range (bytes) form a buffer resource in
+// SGPRs, a lane needs one VGPR of offset per load (not a 64-bit address),
+// and an offset beyond the range reads 0.
+__device__ __forceinline__ const char *uniform_ptr(const char *p)
+{
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (const char *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *base, uint32_t bytes)
+{
+	// third dword: the raw-buffer format bits of gfx9 targets
+	return __builtin_amdgcn_make_buffer_rsrc(
+		(void *)uniform_ptr(static_cast<const char *>(base)), (short)0,
+		(int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ float buf_bf16(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	return __uint_as_float(uint32_t(__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)) << 16);
+}
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+	const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+	return __longlong_as_double((long long)(((uint64_t)v[1] << 32) | v[0]));
+}
+__device__ __forceinline__ void buf_f64x2(__amdgpu_buffer_rsrc_t r, uint32_t off, double &a,
+					  double &b)
+{
+	typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+	const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+	a = __longlong_as_double((long long)(((uint64_t)v[1] << 32) | v[0]));
+	b = __longlong_as_double((long long)(((uint64_t)v[3] << 32) | v[2]));
+}
+__device__ __forceinline__ float4 buf_f32x4(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+	const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+	return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+			   __uint_as_float(v[3]));
+}
+
 constexpr int kTransposeSplits = 8;
 
 // xT[b][n][p] = x[b][p][n] for every block (B interleaved vectors; one

@@ -175,39 +175,14 @@ template <typename S> __device__ __forceinline__ S unpack_words(uint32_t w)
 	return s;
 }
 
-// Loads at a uniform base plus a 32-bit per-lane byte offset, as raw
-// buffer loads: the base and the range (bytes) form a buffer resource in
-// SGPRs, the lane needs one VGPR of offset per load, and an offset beyond
-// the range reads 0
-__device__ __forceinline__ const char *uniform_ptr(const char *p)
-{
-	const uint64_t v = (uint64_t)p;
-	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-	return (const char *)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const char *base, uint32_t bytes)
-{
-	// third dword: the raw-buffer format bits of gfx9 targets
-	return __builtin_amdgcn_make_buffer_rsrc((void *)uniform_ptr(base), (short)0,
-						 (int)__builtin_amdgcn_readfirstlane(bytes),
-						 0x00020000);
-}
-__device__ __forceinline__ float4 ld4_at(__amdgpu_buffer_rsrc_t r, uint32_t off)
-{
-	typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-	const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
-	return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
-			   __uint_as_float(v[3]));
-}
 template <typename VT> __device__ __forceinline__ float elem_at(__amdgpu_buffer_rsrc_t r, uint32_t off);
 template <> __device__ __forceinline__ float elem_at<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-	return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+	return buf_f32(r, off);
 }
 template <> __device__ __forceinline__ float elem_at<uint16_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-	return __uint_as_float(uint32_t(__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)) << 16);
+	return buf_bf16(r, off);
 }
 
 template <typename VT> struct Load;
@@ -342,7 +317,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					for (int c = 0; c < kLaneEntries; c++) {
 						if (kFull || c < nc) {
 							va[k][c] = elem_at<VT>(vr, colb[c]);
-							xa[k][c] = ld4_at(xr, xob[c]);
+							xa[k][c] = buf_f32x4(xr, xob[c]);
 						} else {
 							va[k][c] = 0.f;
 							xa[k][c] = make_float4(0.f, 0.f, 0.f, 0.f);
