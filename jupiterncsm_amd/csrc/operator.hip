@@ -125,7 +125,6 @@ template <> struct Elem<double> {
 	using acc_t = double;
 	using x_t = double;
 	__device__ static double load(const double *p, int64_t i) { return p[i]; }
-	__device__ static double bload(__amdgpu_buffer_rsrc_t r, uint32_t off) { return buf_f64(r, off); }
 	__device__ static void load2(const double *p, int64_t i, double &a, double &b)
 	{
 		const double2 v = *reinterpret_cast<const double2 *>(p + i);
@@ -137,7 +136,6 @@ template <> struct Elem<float> {
 	using acc_t = float;
 	using x_t = float;
 	__device__ static float load(const float *p, int64_t i) { return p[i]; }
-	__device__ static float bload(__amdgpu_buffer_rsrc_t r, uint32_t off) { return buf_f32(r, off); }
 };
 template <> struct Elem<uint16_t> {
 	using acc_t = float;
@@ -146,18 +144,7 @@ template <> struct Elem<uint16_t> {
 	{
 		return __uint_as_float(uint32_t(p[i]) << 16);
 	}
-	__device__ static float bload(__amdgpu_buffer_rsrc_t r, uint32_t off) { return buf_bf16(r, off); }
 };
-
-// an input value at a byte offset of a buffer resource
-__device__ __forceinline__ double xload(__amdgpu_buffer_rsrc_t r, uint32_t off, const double *)
-{
-	return buf_f64(r, off);
-}
-__device__ __forceinline__ float xload(__amdgpu_buffer_rsrc_t r, uint32_t off, const float *)
-{
-	return buf_f32(r, off);
-}
 
 // Owner-computes tile kernel.  A workgroup owns an output tile: the rows
 // p_o in [po_begin, po_end) of one basis block, all columns, held in LDS.
@@ -228,9 +215,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += 128 * KP) {
 				const int nc = (int)min((uint32_t)KP, (T.e1 - e0 + 127u) / 128u);
 				bool vl[KP];
-				// byte offsets of the lane's element pair (from the
-				// element row) and of its input value (from the input row)
-				uint32_t colb[KP], xinb[KP], outa[KP], outb[KP];
+				uint32_t col[KP], xin[KP], outa[KP], outb[KP];
 				acc_t sga[KP], sgb[KP], acca[KP], accb[KP];
 #pragma unroll
 				for (int c = 0; c < KP; c++) {
@@ -240,8 +225,8 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					vl[c] = e < T.e1;
 					const uint32_t k = I.n_base + (vl[c] ? e : T.e0);
 					const uint2 ea = ent[k], eb = ent[k + 1];
-					colb[c] = (k - I.n_base) * (uint32_t)sizeof(VT);
-					xinb[c] = ea.x * (uint32_t)sizeof(*xblk);
+					col[c] = k - I.n_base;
+					xin[c] = ea.x;
 					outa[c] = keys[k];
 					outb[c] = keys[k + 1];
 					sga[c] = apply_sign(sc, ea.y);
@@ -275,19 +260,13 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						}
 						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-						// element and input rows as buffer resources; a
-						// lane's column (compact index or lane-list
-						// position) is below the lane list's length
-						const __amdgpu_buffer_rsrc_t vr = rsrc_of(
-							vblk + (size_t)(dn * (py & kIndexMask)),
-							I.n_len * (uint32_t)sizeof(VT));
-						const __amdgpu_buffer_rsrc_t xr = rsrc_of(
-							xblk + (size_t)(nn_in * px), nn_in * (uint32_t)sizeof(*xblk));
+						const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+						const auto *xcol = xblk + (size_t)(nn_in * px);
 						acc_t va[KP], vb[KP], xa[KP];
 #pragma unroll
 						for (int c = 0; c < NC; c++) {
-							buf_f64x2(vr, colb[c], va[c], vb[c]);
-							xa[c] = acc_t(xload(xr, xinb[c], xblk));
+							Elem<VT>::load2(vrow, col[c], va[c], vb[c]);
+							xa[c] = acc_t(xcol[xin[c]]);
 						}
 #pragma unroll
 						for (int c = 0; c < NC; c++) {
@@ -331,7 +310,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 				// the loads of empty chunks are skipped
 				const int nc = (int)min((uint32_t)kLaneEntries, (T.e1 - e0 + 63u) / 64u);
 				bool vl[kLaneEntries];
-				uint32_t colb[kLaneEntries], xinb[kLaneEntries], outc[kLaneEntries];
+				uint32_t col[kLaneEntries], xin[kLaneEntries], outc[kLaneEntries];
 				acc_t sg[kLaneEntries], acc[kLaneEntries];
 #pragma unroll
 				for (int c = 0; c < kLaneEntries; c++) {
@@ -341,9 +320,8 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					const uint2 en = ent[k];
 					// element column: the lane entry's compact index, or
 					// its position in the lane list for expanded rows
-					colb[c] = (I.expanded ? k - I.n_base : en.y & kIndexMask) *
-						  (uint32_t)sizeof(VT);
-					xinb[c] = en.x * (uint32_t)sizeof(*xblk);
+					col[c] = I.expanded ? k - I.n_base : en.y & kIndexMask;
+					xin[c] = en.x;
 					outc[c] = keys[k];
 					sg[c] = apply_sign(sc, en.y);
 					acc[c] = 0;
@@ -377,20 +355,14 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 						}
 						const uint32_t px = __builtin_amdgcn_readlane(pl.x, t);
 						const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
-						// element and input rows as buffer resources; a
-						// lane's column (compact index or lane-list
-						// position) is below the lane list's length
-						const __amdgpu_buffer_rsrc_t vr = rsrc_of(
-							vblk + (size_t)(dn * (py & kIndexMask)),
-							I.n_len * (uint32_t)sizeof(VT));
-						const __amdgpu_buffer_rsrc_t xr = rsrc_of(
-							xblk + (size_t)(nn_in * px), nn_in * (uint32_t)sizeof(*xblk));
+						const VT *vrow = vblk + (size_t)(dn * (py & kIndexMask));
+						const auto *xcol = xblk + (size_t)(nn_in * px);
 						acc_t va[kLaneEntries], xa[kLaneEntries];
 #pragma unroll
 						for (int c = 0; c < NL; c++) {
 							if (NC || c < nc) {
-								va[c] = Elem<VT>::bload(vr, colb[c]);
-								xa[c] = acc_t(xload(xr, xinb[c], xblk));
+								va[c] = Elem<VT>::load(vrow, col[c]);
+								xa[c] = acc_t(xcol[xin[c]]);
 							} else {
 								va[c] = 0;
 								xa[c] = 0;
@@ -799,6 +771,25 @@ int make_plan(const bacchus_hip_operator *op, const PlanParams &pp, Plan &out)
 			load[k] += g.first;
 			bins[k].insert(bins[k].end(), g.second.begin(), g.second.end());
 		}
+		// within an XCD: the tiles of one output block (they read the same
+		// element sub-blocks, split by rows) one after another, so their
+		// shared element rows are still in the XCD's L2; blocks heaviest
+		// first (BACCHUS_HIP_TILE_ORDER=0: every tile heaviest first)
+		const char *ord = getenv("BACCHUS_HIP_TILE_ORDER");
+		if (!(ord && atoi(ord) == 0))
+			for (auto &b : bins) {
+				std::map<std::pair<int, int64_t>, double> bc;
+				for (size_t i : b)
+					bc[{out.tiles[i].space, out.tiles[i].y_off}] += out.cost[i];
+				std::stable_sort(b.begin(), b.end(), [&](size_t u, size_t v) {
+					const TileWork &a = out.tiles[u], &c = out.tiles[v];
+					const double ca = bc[{a.space, a.y_off}], cc = bc[{c.space, c.y_off}];
+					if (ca != cc)
+						return ca > cc;
+					return std::make_tuple(a.space, a.y_off, a.vi_begin, a.po_begin) <
+					       std::make_tuple(c.space, c.y_off, c.vi_begin, c.po_begin);
+				});
+			}
 		size_t len = 0;
 		for (auto &b : bins)
 			len = std::max(len, b.size());
