@@ -1041,12 +1041,16 @@ int build_plan(bacchus_hip_batch_t *bt)
 		int64_t y_off;
 		int po_begin;
 	};
-	// input groups, one work queue per XCD (balanced over all operators)
+	// input groups (balanced over all operators), one work queue per XCD;
+	// with more groups than XCDs (BACCHUS_HIP_BATCH_GROUPS), group g goes to
+	// queue g mod 8 (measured: 16 groups C3 block 6.53 ms, C4 2.13 ms; 8
+	// groups 6.60 and 0.76 ms)
 	int groups = kXcds;
 	if (const char *ge = getenv("BACCHUS_HIP_BATCH_GROUPS"))
-		groups = std::max(1, std::min(kXcds, atoi(ge)));
+		groups = std::max(1, std::min(256, atoi(ge)));
+	const int nqueues = std::min(groups, kXcds);
 	std::vector<double> group_load(groups, 0.0);
-	std::vector<std::vector<Cand>> queues(groups);
+	std::vector<std::vector<Cand>> queues(nqueues);
 	int64_t max_words = 1;
 	for (int i = 0; i < nops; i++) {
 		const bacchus_hip_operator_t *op = bt->ops[i];
@@ -1106,7 +1110,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 			while ((1 << lbt) < v)
 				lbt++;
 			for (int b0 = 0; b0 < Bp; b0 += v)
-				queues[plan.tiles[t].group].push_back(
+				queues[plan.tiles[t].group % nqueues].push_back(
 					Cand{plan.cost[t], BatchItem{i, (int32_t)t, b0, lbt}, plan.tiles[t].space,
 					     plan.tiles[t].y_off, plan.tiles[t].po_begin});
 		}
@@ -1142,7 +1146,7 @@ int build_plan(bacchus_hip_batch_t *bt)
 		qbegin.push_back((int)items.size());
 	}
 	bt->num_items = (int64_t)items.size();
-	bt->nq = groups;
+	bt->nq = nqueues;
 	BACCHUS_HIP_CHECK(hipMalloc(&bt->d_items, std::max<size_t>(16, items.size() * sizeof(BatchItem))));
 	if (!items.empty())
 		BACCHUS_HIP_CHECK(hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(BatchItem),

@@ -1624,10 +1624,11 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	pp.skip_np = op->np_path;
 	if (const char *xe = getenv("BACCHUS_HIP_XCDS"))
 		pp.xcds = atoi(xe);
-	// tiles grouped per XCD by the input blocks they read (measured 4.1
-	// against 4.3 ms with the grouping by output block on the 3+3 Nmax 6
-	// FP64 matvec)
-	pp.in_groups = kXcds;
+	// tiles grouped by the input blocks they read, eight groups per XCD
+	// (3+3 Nmax 6 FP64 matvec: 4.3 ms grouped by output block, 3.80 ms
+	// with one input group per XCD, 3.39 ms with 64; 16, 32 and 48 groups
+	// 3.62, 3.48, 3.45 ms; 96 and more 3.51 ms)
+	pp.in_groups = 8 * kXcds;
 	if (const char *ge = getenv("BACCHUS_HIP_IN_GROUPS"))
 		pp.in_groups = atoi(ge);
 	Plan plan;
