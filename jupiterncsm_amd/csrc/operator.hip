@@ -1110,12 +1110,23 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		if (I.type == 3 && p->matrix_force && p->matrix_force[I.matrix_block] == 3)
 			lane_of_3body[swapped[i] ? I.proton_list : I.neutron_list] = 1;
 	}
+	// uses per element up to which a lane list is expanded: 2 for 2-body
+	// lists, 3 for 3-body ones (BACCHUS_HIP_EXPAND=<2-body>:<3-body> to
+	// measure others; 0 expands nothing)
+	int64_t expand2 = 2, expand3 = 3;
+	if (const char *ex = getenv("BACCHUS_HIP_EXPAND")) {
+		long a = 2, b = 3;
+		if (sscanf(ex, "%ld:%ld", &a, &b) >= 1) {
+			expand2 = a;
+			expand3 = strchr(ex, ':') ? b : a;
+		}
+	}
 	for (int64_t i = 0; i < ni; i++) {
 		const bacchus_hip_instruction_t &I = p->instructions[i];
 		if (I.type != 3)
 			continue;
 		const int32_t l = swapped[i] ? I.proton_list : I.neutron_list;
-		expanded[l] = list_len(l) <= (lane_of_3body[l] ? 3 : 2) * (int64_t)mvals[l].size();
+		expanded[l] = list_len(l) <= (lane_of_3body[l] ? expand3 : expand2) * (int64_t)mvals[l].size();
 	}
 	auto lane_list = [&](const bacchus_hip_instruction_t &I, int64_t i) -> int32_t {
 		return swapped[i] ? I.proton_list : I.neutron_list;
