@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -91,6 +92,20 @@ template <typename F> void parallel_for(size_t n, F &&fn)
 	if (failure)
 		std::rethrow_exception(failure);
 }
+
+// host stage times of operator creation on stderr under BACCHUS_HIP_VERBOSE
+struct StageClock {
+	bool on = getenv("BACCHUS_HIP_VERBOSE") != nullptr;
+	std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+	void mark(const char *what)
+	{
+		const auto now = std::chrono::steady_clock::now();
+		if (on)
+			fprintf(stderr, "[bacchus] operator create: %s %.3f s\n", what,
+				std::chrono::duration<double>(now - t).count());
+		t = now;
+	}
+};
 
 // allocator whose resize() leaves new elements uninitialized: every range
 // the element table opens is written in full afterwards, so zeroing it
@@ -955,6 +970,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	op->dimension = op->block_off[nb];
 	auto fail = [](int code) { return code; };
 
+	StageClock clk;
 	// --- host validation: every index a kernel will touch is in bounds ---
 	std::vector<ListUse> use(nl);
 	std::vector<int64_t> vlen(nv), vdn(nv);
@@ -1056,6 +1072,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 						 (long long)use[l].n_out));
 	}
 
+	clk.mark("validation");
 	// --- compact element indices: every used list gets the sorted set of
 	// element indices it references; entries carry the position in that
 	// set (phase bit kept).  Each (element block, neutron list, proton list)
@@ -1080,6 +1097,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		const uint32_t c = (uint32_t)(std::lower_bound(mv.begin(), mv.end(), v) - mv.begin());
 		return c | ((uint32_t)m & 0x80000000u);
 	};
+	clk.mark("compact element indices");
 	// --- which instructions run in the transposed space (every block read
 	// and written as [neutron][proton], protons fastest): NP instructions
 	// whose proton list is the longer one (the kernel's lanes then walk the
@@ -1185,6 +1203,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		}
 	});
 
+	clk.mark("transposed space, element sub-blocks");
 	// --- CSR copies of every used list, by output and by input index,
 	// each row ordered by element index; keys[k] = row of entry k.  Every
 	// (list, direction) gets its place first; the copies are then built by
@@ -1418,6 +1437,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		W.reserve(W.size() + (size_t)extra);
 	}
 
+	clk.mark("CSR and lane copies");
 	// --- virtual instructions (each direction) grouped by output block;
 	// slots [0, nb) are the normal space, [nb, 2 nb) the transposed one ---
 	std::vector<std::vector<TInstr>> per_block(2 * nb);
@@ -1500,6 +1520,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	}
 	op->compact_elements = (int64_t)W.size();
 
+	clk.mark("virtual instructions, expanded rows");
 	// --- NP virtual instructions of one output block that share their
 	// lane copy, force class and input block are merged into one: their
 	// row-side entries, by output row, in a new CSR whose entries carry the
@@ -1578,6 +1599,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		}
 	}
 
+	clk.mark("merged row-side lists");
 	// --- host copies for the planner; virtual instructions of a block
 	// sorted by element sub-block so neighbouring tasks share elements ---
 	op->blk_vi.assign(2 * nb + 1, 0);
@@ -1664,6 +1686,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	op->num_tinstr = (int64_t)tinstr.size();
 	op->tile_lds_bytes = (size_t)plan.max_tile_elems * sizeof(double);
 
+	clk.mark("plan");
 	// --- statistics: compulsory bytes of referenced arrays ---
 	std::vector<char> vref(nv, 0);
 	for (int64_t i = 0; i < ni; i++)
@@ -1678,6 +1701,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 				16.0 * double(op->dimension);
 	op->flops = 2.0 * mults;
 
+	clk.mark("statistics");
 	// --- upload ---
 	auto upload = [&](void **dst, const void *src, size_t bytes) -> int {
 		if (bytes == 0)
@@ -1737,6 +1761,7 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 		    (rc = upload((void **)&op->d_blk_nn, bnn.data(), nb * sizeof(int32_t))))
 			return fail(rc);
 	}
+	clk.mark("upload");
 	*out = guard.release();
 	return 0;
 }
