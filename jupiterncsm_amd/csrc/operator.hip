@@ -1664,6 +1664,9 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	pp.in_groups = 8 * kXcds;
 	if (const char *ge = getenv("BACCHUS_HIP_IN_GROUPS"))
 		pp.in_groups = atoi(ge);
+	// LDS doubles per tile (measurement; the kernel's LDS follows the plan)
+	if (const char *te = getenv("BACCHUS_HIP_TILE_ELEMS"))
+		pp.tile_elems = std::max<int64_t>(1024, std::min<int64_t>(16384, atoll(te)));
 	Plan plan;
 	if (int rc = make_plan(op, pp, plan))
 		return fail(rc);
