@@ -247,6 +247,19 @@ def main():
         by_precision[pname] = [float(v) for v in np.abs(rx.eigenvalues - targets)
                                / np.abs(targets)]
         log(f"{pname} storage: relative eigenvalue errors {by_precision[pname]}")
+    # a mixed store: the 3-body elements (sigma 0.1 MeV) as BF16 values, the 1-
+    # and 2-body ones as FP32 (held here in an FP32 operator, so the numerics
+    # are those of the mixed store, not its bytes)
+    opm = mixed_operator(inputs, (3,))
+    rx = bacchus.diagonalize(opm, args.iterations, num_eigenvalues=args.eigenvalues,
+                             start=bacchus.START_GIVEN, start_vector=sv,
+                             reorthogonalization=bacchus.REORTH_FULL,
+                             convergence=bacchus.CONVERGE_NONE)
+    opm.close()
+    by_precision["bf16_3body_fp32_rest"] = [float(v) for v in np.abs(rx.eigenvalues - targets)
+                                            / np.abs(targets)]
+    log(f"BF16 3-body / FP32 1- and 2-body elements: relative eigenvalue errors "
+        f"{by_precision['bf16_3body_fp32_rest']}")
 
     # --- learned start: trained on batched solves of the family (untimed) -
     model = fam = bt = mblock = block_mv = None
@@ -580,6 +593,36 @@ def small_systems(args, cores, cpu=True):
         out[name] = rec
         log(f"{name}: {rec}")
     return out
+
+
+def bf16_values(a):
+    """FP64 array -> the values of its BF16 rounding (round to nearest even
+    of the FP32 value), as the library stores BF16 elements."""
+    import numpy as np
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    finite = (u & 0x7F800000) != 0x7F800000
+    u = np.where(finite, u + 0x7FFF + ((u >> 16) & 1), u)
+    return ((u >> 16) << 16).astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def mixed_operator(inputs, bf16_forces):
+    """An FP32 operator whose elements of the given force classes are BF16
+    values (the numerics of a store with those classes in BF16)."""
+    import numpy as np
+    from jupiterncsm_amd import bacchus
+    elems = np.array(inputs.matrix_elements, dtype=np.float64, copy=True)
+    offs = inputs.matrix_offsets
+    for k, force in enumerate(inputs.matrix_shapes[:, 2]):
+        if int(force) in bf16_forces:
+            elems[offs[k]:offs[k + 1]] = bf16_values(elems[offs[k]:offs[k + 1]])
+    ins = inputs.instructions
+    raw = np.stack([ins[f] for f in ins.dtype.names], axis=1).astype(np.int32)
+    op = bacchus.Operator.from_arrays(inputs.blocks[:, 4], inputs.blocks[:, 5],
+                                      inputs.list_offsets, inputs.list_entries, offs,
+                                      inputs.matrix_shapes[:, 0], inputs.matrix_shapes[:, 2],
+                                      elems, raw, bacchus.FP32)
+    del elems
+    return op
 
 
 def batch_without_mfma(ops, counts):
