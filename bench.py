@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--start-mix", type=float, default=1e-4,
                     help="relative size of the random component mixed into the MLP start")
     ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
+    ap.add_argument("--projected-basis", type=int, default=128,
+                    help="stored band Ritz vectors the projected start's basis is made of")
     ap.add_argument("--block-mlp-steps", type=int, default=8000,
                     help="training steps of each MLP of the learned start block (0: none)")
     ap.add_argument("--max-blocks", type=int, default=16)
@@ -270,7 +272,8 @@ def main():
         pts = L.family_points(args.train_points, 1)
         sol = L.solve_family(op32, pts, iterations=args.train_iterations,
                              num_eigenvalues=args.eigenvalues, target="band",
-                             band=args.eigenvalues, keep_ritz=-(-64 // args.eigenvalues),
+                             band=args.eigenvalues,
+                             keep_ritz=-(-args.projected_basis // args.eigenvalues),
                              band_references=args.block if args.block_mlp_steps else 0)
         model = L.fit(sol, basis_size=64, steps=3000, target="band", precision=MLP_PRECISION)
         if args.block_mlp_steps:
@@ -279,10 +282,15 @@ def main():
             log(f"MLP start block: {args.block} networks, losses "
                 f"{min(mblock.train_losses):.2e}..{max(mblock.train_losses):.2e}")
         t1 = time.time()
-        fam = L.ProjectedFamily.build(op64, model.basis, model.rank)
+        # the projected family's own stored basis: the band Ritz vectors of
+        # the first points (more of them than the networks' 64 fit the band
+        # at a new point better at the same numerical rank)
+        fam_basis, fam_rank = L.orthonormal_basis(sol.ritz[:args.projected_basis].contiguous())
+        fam = L.ProjectedFamily.build(op64, fam_basis, fam_rank)
         train_s = time.time() - t0
         log(f"learned start: {args.train_points} batched family solves "
-            f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, basis rank {model.rank}, "
+            f"({sol.seconds:.1f}s), MLP loss {model.train_loss:.2e}, MLP basis rank {model.rank}, "
+            f"projected basis {args.projected_basis} vectors of rank {fam_rank}, "
             f"projections {time.time() - t1:.1f}s, total {train_s:.1f}s")
         bt = bacchus.Batch([op32], [args.block])
         bt.set_scales(0, np.ones((args.block, 3)))
@@ -455,9 +463,9 @@ def main():
     if fam is not None:
         fast_desc = (f"fp32 elements; block Lanczos ({args.block} vectors per batched launch, "
                      "FP64 inner products and block tridiagonal solve, stops on its residual "
-                     "bound); start block = Ritz vectors of H(c1,c2) in a 64-vector stored "
-                     "basis of band eigenvectors from "
-                     f"{args.train_points} batched family solves")
+                     "bound; P-form products on the matrix cores); start block = Ritz vectors "
+                     f"of H(c1,c2) in a stored basis of {args.projected_basis} band Ritz vectors "
+                     f"(rank {fam.rank}) from {args.train_points} batched family solves")
     else:
         fast_desc = "fp32 elements + fp64 single-vector lanczos, full reorth, random start"
     line = {
