@@ -24,7 +24,7 @@
 namespace bacchus {
 namespace {
 
-constexpr int kInnerChunks = 128;
+constexpr int kInnerChunks = 512;
 constexpr int kMaxBlock = 16;
 constexpr int kMaxKrylov = 256;
 
@@ -37,32 +37,70 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t z)
 }
 
 // partial[(j * chunks + c) * b * b + p * b + q] = sum over the rows r of
-// chunk c of Q_j[r][p] * W[r][q]; Q holds nblk blocks [D][b]
+// chunk c of Q_j[r][p] * W[r][q]; Q holds nblk blocks [D][b].  Thread
+// (g, q) of the block's 256 / b row groups walks rows g, g + 256 / b, ...
+// of the chunk with the b sums of column q in registers (its W loads and
+// the row's Q values are contiguous across the wave), then the row groups'
+// sums are added in LDS in a fixed order.
 __global__ __launch_bounds__(256) void block_inner(const float *__restrict__ Q,
 						   const float *__restrict__ W, int64_t D, int b,
 						   double *__restrict__ partial)
 {
-	__shared__ double sh[256];
+	__shared__ double sh[256 * kMaxBlock];
 	const int j = blockIdx.y;
 	const int bb = b * b;
-	const int groups = 256 / bb;
+	const int G = 256 / b;
 	const int t = threadIdx.x;
-	const int g = t / bb, pq = t % bb, p = pq / b, q = pq % b;
+	const int g = t / b, q = t - g * b;
 	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
 	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
 	const float *Qj = Q + (int64_t)j * D * b;
-	double s = 0;
-	if (g < groups)
-		for (int64_t r = r0 + g; r < r1; r += groups)
-			s += (double)Qj[r * b + p] * (double)W[r * b + q];
-	sh[t] = s;
+	double s[kMaxBlock];
+#pragma unroll
+	for (int p = 0; p < kMaxBlock; p++)
+		s[p] = 0.0;
+	if (g < G)
+		for (int64_t r = r0 + g; r < r1; r += G) {
+			const double w = (double)W[r * b + q];
+			const float *row = Qj + r * b;
+#pragma unroll
+			for (int p = 0; p < kMaxBlock; p++)
+				if (p < b)
+					s[p] += (double)row[p] * w;
+		}
+	if (g < G)
+#pragma unroll
+		for (int p = 0; p < kMaxBlock; p++)
+			if (p < b)
+				sh[(g * b + p) * b + q] = s[p];
 	__syncthreads();
 	if (t < bb) {
 		double v = 0;
-		for (int k = 0; k < groups; k++)
-			v += sh[t + k * bb];
+		for (int k = 0; k < G; k++)
+			v += sh[k * bb + t];
 		partial[((int64_t)j * gridDim.x + blockIdx.x) * bb + t] = v;
 	}
+}
+
+// out[e] = sum over the chunks of partial[(j * chunks + c) * bb + e'] for
+// e = j * bb + e': one block per output, thread t adding chunks t, t + 256,
+// ..., then a fixed shuffle and LDS tree (the same order on every call)
+__global__ __launch_bounds__(256) void block_inner_reduce(const double *__restrict__ partial,
+							  int chunks, int bb, double *__restrict__ out)
+{
+	__shared__ double sh[4];
+	const int e = blockIdx.x;
+	const int j = e / bb, k = e - j * bb;
+	double v = 0;
+	for (int c = threadIdx.x; c < chunks; c += blockDim.x)
+		v += partial[((int64_t)j * chunks + c) * bb + k];
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o, 64);
+	if ((threadIdx.x & 63) == 0)
+		sh[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		out[e] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // out[r][q] = in[r][q] (or 0) + sum_{j, p} Q_j[r][p] C[(j b + p) m + q] for
@@ -180,7 +218,7 @@ struct Block {
 	float *Y = nullptr;      // scratch       [D][b]
 	double *dC = nullptr;    // combination coefficients
 	double *dpart = nullptr; // inner-product partial sums
-	std::vector<double> hpart;
+	double *dsum = nullptr;  // their sums over the chunks
 	double matvec_ms = 0;
 
 	float *blk(int j) const { return Q + (int64_t)j * D * b; }
@@ -193,8 +231,9 @@ struct Block {
 		const size_t nq = vec * (max_blocks + 1);
 		const size_t nc = up(sizeof(double) * (size_t)(max_blocks + 1) * b * kMaxBlock);
 		const size_t np = up(sizeof(double) * (size_t)(max_blocks + 1) * kInnerChunks * b * b);
+		const size_t ns = up(sizeof(double) * (size_t)(max_blocks + 1) * b * b);
 		void *ws = nullptr;
-		if (int rc = batch_workspace(batch, nq + 3 * vec + nc + np, &ws))
+		if (int rc = batch_workspace(batch, nq + 3 * vec + nc + np + ns, &ws))
 			return rc;
 		char *p = static_cast<char *>(ws);
 		Q = reinterpret_cast<float *>(p);
@@ -203,6 +242,7 @@ struct Block {
 		Y = reinterpret_cast<float *>(p + nq + 2 * vec);
 		dC = reinterpret_cast<double *>(p + nq + 3 * vec);
 		dpart = reinterpret_cast<double *>(p + nq + 3 * vec + nc);
+		dsum = reinterpret_cast<double *>(p + nq + 3 * vec + nc + np);
 		return 0;
 	}
 
@@ -212,17 +252,11 @@ struct Block {
 		const int bb = b * b;
 		hipLaunchKernelGGL(block_inner, dim3(kInnerChunks, (unsigned)nblk), dim3(256), 0, nullptr, Qs, V,
 				   D, b, dpart);
-		hpart.resize((size_t)nblk * kInnerChunks * bb);
-		BACCHUS_HIP_CHECK(hipMemcpy(hpart.data(), dpart, hpart.size() * sizeof(double),
-					    hipMemcpyDeviceToHost));
-		C.assign((size_t)nblk * bb, 0.0);
-		for (int j = 0; j < nblk; j++)
-			for (int e = 0; e < bb; e++) {
-				double s = 0;
-				for (int c = 0; c < kInnerChunks; c++)
-					s += hpart[((size_t)j * kInnerChunks + c) * bb + e];
-				C[(size_t)j * bb + e] = s;
-			}
+		const int n = nblk * bb;
+		hipLaunchKernelGGL(block_inner_reduce, dim3((unsigned)n), dim3(256), 0, nullptr, dpart,
+				   kInnerChunks, bb, dsum);
+		C.resize((size_t)n);
+		BACCHUS_HIP_CHECK(hipMemcpy(C.data(), dsum, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
 		return 0;
 	}
 
@@ -258,12 +292,15 @@ struct Block {
 	// V (orthogonal to the stored blocks) = out R, out orthonormal: SVQB
 	// (the Gram matrix's eigensystem); directions whose Gram eigenvalue is
 	// below floor get seeded random columns, orthogonalized, with zero
-	// rows in R.  V, out and Y are distinct buffers.
+	// rows in R.  V, out and Y are distinct buffers; gram, when given, is
+	// V's Gram matrix.
 	int svqb(int nblk, float *V, float *out, std::vector<double> &R, double floor, uint64_t seed,
-		 double *cond = nullptr)
+		 double *cond = nullptr, const std::vector<double> *gram = nullptr)
 	{
 		std::vector<double> G, lam, U;
-		if (int rc = inner(V, 1, V, G))
+		if (gram)
+			G = *gram;
+		else if (int rc = inner(V, 1, V, G))
 			return rc;
 		jacobi_eigen(b, G, lam, U);
 		if (cond)
@@ -364,7 +401,8 @@ struct Block {
 		coef.assign((size_t)nblk * b * b, 0.0);
 		if (int rc = project_out(nblk, V, &coef))
 			return rc;
-		return svqb(nblk, V, out, R, floor, seed, cond);
+		// with no stored blocks V is unchanged and G0 is its Gram matrix
+		return svqb(nblk, V, out, R, floor, seed, cond, nblk == 0 ? &G0 : nullptr);
 	}
 
 	// The second round on out (X is its scratch output): out = Q c2 +

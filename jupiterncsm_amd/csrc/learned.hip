@@ -134,6 +134,34 @@ __global__ void transform_basis(const float *__restrict__ U, int n, int64_t D,
 	out[(int64_t)blockIdx.y * D + r] = (float)v;
 }
 
+// the same for at most kFewOutputs outputs, one thread per component
+// reading U once for all of them (the same sums, in the same order)
+constexpr int kFewOutputs = 16;
+__global__ __launch_bounds__(256) void transform_basis_few(const float *__restrict__ U, int n,
+							   int64_t D, const double *__restrict__ C,
+							   int m, float *__restrict__ out)
+{
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= D)
+		return;
+	double acc[kFewOutputs];
+#pragma unroll
+	for (int q = 0; q < kFewOutputs; q++)
+		acc[q] = 0.0;
+#pragma unroll 8
+	for (int i = 0; i < n; i++) {
+		const double u = (double)U[(int64_t)i * D + r];
+#pragma unroll
+		for (int q = 0; q < kFewOutputs; q++)
+			if (q < m)
+				acc[q] += C[(int64_t)q * n + i] * u;
+	}
+#pragma unroll
+	for (int q = 0; q < kFewOutputs; q++)
+		if (q < m)
+			out[(int64_t)q * D + r] = (float)acc[q];
+}
+
 // one block: ss = |out|^2 in a fixed order, then out /= sqrt(ss)
 __global__ __launch_bounds__(256) void normalize_vector(double *out, int64_t D)
 {
@@ -255,15 +283,34 @@ int bacchus_hip_basis_transform(const float *U, int64_t n, int64_t D, const doub
 {
 	if (!U || !C || !out || n <= 0 || D <= 0 || r <= 0 || r > 65535)
 		return BACCHUS_FAIL("bad arguments");
-	double *dc = nullptr;
-	BACCHUS_HIP_CHECK(hipMalloc(&dc, (size_t)r * n * sizeof(double)));
-	hipError_t e = hipMemcpy(dc, C, (size_t)r * n * sizeof(double), hipMemcpyHostToDevice);
-	if (e == hipSuccess) {
+	// the coefficients go to a device buffer kept per device (grown when a
+	// call needs more); calls hold the lock until their kernel has ended
+	static std::mutex mu;
+	static double *buf[kMaxDevices] = {};
+	static size_t cap[kMaxDevices] = {};
+	std::lock_guard<std::mutex> lock(mu);
+	int dev = 0;
+	BACCHUS_HIP_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		return BACCHUS_FAIL("device %d beyond %d", dev, kMaxDevices);
+	const size_t need = (size_t)r * n * sizeof(double);
+	if (cap[dev] < need) {
+		if (buf[dev])
+			BACCHUS_HIP_CHECK(hipFree(buf[dev]));
+		buf[dev] = nullptr;
+		cap[dev] = 0;
+		BACCHUS_HIP_CHECK(hipMalloc(&buf[dev], need));
+		cap[dev] = need;
+	}
+	double *dc = buf[dev];
+	BACCHUS_HIP_CHECK(hipMemcpy(dc, C, need, hipMemcpyHostToDevice));
+	if (r <= kFewOutputs)
+		hipLaunchKernelGGL(transform_basis_few, dim3((unsigned)((D + 255) / 256)), dim3(256), 0,
+				   nullptr, U, (int)n, D, dc, (int)r, out);
+	else
 		hipLaunchKernelGGL(transform_basis, dim3((unsigned)((D + 255) / 256), (unsigned)r), dim3(256),
 				   0, nullptr, U, (int)n, D, dc, out);
-		e = hipDeviceSynchronize();
-	}
-	(void)hipFree(dc);
+	const hipError_t e = hipDeviceSynchronize();
 	if (e != hipSuccess)
 		return BACCHUS_FAIL("transform: %s", hipGetErrorString(e));
 	return 0;

@@ -332,9 +332,11 @@ class ProjectedFamily:
         """Lowest k Ritz values and coefficient vectors (rank, k) over the
         basis rows, normalized in the N inner product."""
         M = self.parts[0] + c1 * self.parts[1] + c2 * self.parts[2]
-        # N = L L^T; eigen of L^-1 M L^-T
-        L = np.linalg.cholesky(self.gram)
-        Li = np.linalg.inv(L)
+        # N = L L^T; eigen of L^-1 M L^-T (L^-1 is the same for every point)
+        Li = getattr(self, "_gram_cholesky_inverse", None)
+        if Li is None:
+            Li = np.linalg.inv(np.linalg.cholesky(self.gram))
+            self._gram_cholesky_inverse = Li
         A = Li @ M @ Li.T
         w, Z = np.linalg.eigh(0.5 * (A + A.T))
         return w[:k], Li.T @ Z[:, :k]
@@ -343,9 +345,7 @@ class ProjectedFamily:
         """Float32 device block (D, b): the b lowest Ritz vectors of
         H(c1, c2) in the basis span."""
         _, S = self.ritz(c1, c2, b)
-        C = np.zeros((b, self.basis.shape[0]))
-        C[:, :self.rank] = S.T
-        return bacchus.basis_transform(self.basis, C).T.contiguous()
+        return bacchus.basis_transform(self.basis[:self.rank], S.T).T.contiguous()
 
 
 def solve_from(operator, c, start_vector, *, num_eigenvalues, targets, tolerance=1e-6,
