@@ -211,7 +211,7 @@ template <> struct Load<uint16_t> {
 //   N (:20-77) runs in the transposed space as P there.
 // N and P skip elements with |s V| < 1e-12 per vector (s its force-class
 // scale), as the reference does on the element it reads (:50-53, :105-108).
-template <typename VT>
+template <typename VT, int kAhead>
 __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 			   double *tile)
 {
@@ -265,9 +265,10 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 		if (I.type == 3) {
 			// every lane holds kLaneEntries lane-side entries (slot
 			// chunks), so a broadcast row-side entry puts 2 kLaneEntries
-			// independent loads in flight; the loads of row-side entry
-			// t + 1 are issued before the products of entry t, so two
-			// entries' loads overlap (the element rows stream from HBM)
+			// independent loads in flight; the loads of row-side entries
+			// t + 1 .. t + kAhead are issued before the products of entry
+			// t, so kAhead + 1 entries' loads overlap (the element rows
+			// stream from HBM)
 			const VT *vblk = V + I.v_off;
 			const uint32_t dn = (uint32_t)I.dn;
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += kLaneEntries * G) {
@@ -297,8 +298,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				}
 				auto rows_pass = [&](auto full) {
 				constexpr bool kFull = decltype(full)::value;
-				float va[2][kLaneEntries];
-				float4 xa[2][kLaneEntries];
+				float va[kAhead + 1][kLaneEntries];
+				float4 xa[kAhead + 1][kLaneEntries];
 				uint2 pl;
 				// loads of row-side entry t (lane t of pl) into buffer buf
 				auto issue = [&](uint32_t t, auto buf) {
@@ -357,20 +358,43 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				};
 				using B0 = std::integral_constant<int, 0>;
 				using B1 = std::integral_constant<int, 1>;
+				using B2 = std::integral_constant<int, kAhead == 2 ? 2 : 0>;
 				for (uint32_t j0 = T.jb; j0 < T.je; j0 += 64) {
 					const uint32_t cnt = min(64u, T.je - j0);
 					const uint32_t jl = j0 + min((uint32_t)lane, cnt - 1);
 					pl = global_of(o.ent)[jl];
 					const uint32_t prow = global_of(o.keys)[jl];
 					issue(0u, B0{});
-					for (uint32_t t = 0; t < cnt; t += 2) {
-						if (t + 1 < cnt)
-							issue(t + 1, B1{});
-						consume(t, prow, B0{});
-						if (t + 1 < cnt) {
+					if constexpr (kAhead == 1) {
+						for (uint32_t t = 0; t < cnt; t += 2) {
+							if (t + 1 < cnt)
+								issue(t + 1, B1{});
+							consume(t, prow, B0{});
+							if (t + 1 < cnt) {
+								if (t + 2 < cnt)
+									issue(t + 2, B0{});
+								consume(t + 1, prow, B1{});
+							}
+						}
+					} else {
+						// the loads of entries t + 1 and t + 2 in flight
+						// while the products of t are formed
+						if (1 < cnt)
+							issue(1u, B1{});
+						for (uint32_t t = 0; t < cnt; t += 3) {
 							if (t + 2 < cnt)
-								issue(t + 2, B0{});
-							consume(t + 1, prow, B1{});
+								issue(t + 2, B2{});
+							consume(t, prow, B0{});
+							if (t + 1 < cnt) {
+								if (t + 3 < cnt)
+									issue(t + 3, B0{});
+								consume(t + 1, prow, B1{});
+							}
+							if (t + 2 < cnt) {
+								if (t + 4 < cnt)
+									issue(t + 4, B1{});
+								consume(t + 2, prow, B2{});
+							}
 						}
 					}
 				}
@@ -485,6 +509,7 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 // XCD (workgroup i runs on XCD i mod kXcds) until it is exhausted, then
 // from the others in turn.  The queue heads next[q] are cleared by
 // reset_queue before every launch.
+template <int kAhead>
 __device__ __forceinline__ void batch_body(const BatchOp *__restrict__ ops,
 					   const BatchItem *__restrict__ items,
 					   const int *__restrict__ qbegin, int nq, int *__restrict__ next)
@@ -507,28 +532,30 @@ __device__ __forceinline__ void batch_body(const BatchOp *__restrict__ ops,
 			const BatchOp o = ops[bi.op];
 			const TileWork w = o.tiles[bi.tile];
 			if (o.precision == BACCHUS_HIP_BF16)
-				batch_tile<uint16_t>(o, w, bi.b0, bi.lbt, tile);
+				batch_tile<uint16_t, kAhead>(o, w, bi.b0, bi.lbt, tile);
 			else
-				batch_tile<float>(o, w, bi.b0, bi.lbt, tile);
+				batch_tile<float, kAhead>(o, w, bi.b0, bi.lbt, tile);
 			__syncthreads();
 		}
 	}
 }
 
 // The same kernel compiled twice: with the registers the compiler chooses
-// (130 VGPRs, three waves per SIMD) and held to 128 VGPRs for a fourth
-// wave; the autotuner keeps the faster for the batch.
+// and the NP row loop loading two row entries ahead (152 VGPRs, three waves
+// per SIMD; one entry ahead: 6.64 ms for the C3 8-vector block against
+// 6.38 ms), and held to 128 VGPRs for a fourth wave with one entry ahead;
+// the autotuner keeps the faster for the batch.
 __global__ __launch_bounds__(kTileThreads) void batch_kernel(
 	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
 	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
 {
-	batch_body(ops, items, qbegin, nq, next);
+	batch_body<2>(ops, items, qbegin, nq, next);
 }
 __global__ __launch_bounds__(kTileThreads, 4) void batch_kernel_w4(
 	const BatchOp *__restrict__ ops, const BatchItem *__restrict__ items,
 	const int *__restrict__ qbegin, int nq, int *__restrict__ next)
 {
-	batch_body(ops, items, qbegin, nq, next);
+	batch_body<1>(ops, items, qbegin, nq, next);
 }
 
 typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
