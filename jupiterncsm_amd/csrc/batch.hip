@@ -271,6 +271,24 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 			// stream from HBM)
 			const VT *vblk = V + I.v_off;
 			const uint32_t dn = (uint32_t)I.dn;
+			// list position of the lane's entry in chunk c of the pass at
+			// e0 (the task's first entry where the chunk runs past its end)
+			auto lane_k = [&](uint32_t e0, int c) {
+				const uint32_t e = e0 + (uint32_t)(c * G + slot);
+				return I.n_base + (e < T.e1 ? e : T.e0);
+			};
+			// with two row entries ahead, the lane-side entries of the
+			// next pass are loaded while this pass runs its rows
+			uint2 en_nx[kLaneEntries];
+			uint32_t key_nx[kLaneEntries];
+			if constexpr (kAhead == 2) {
+#pragma unroll
+				for (int c = 0; c < kLaneEntries; c++) {
+					const uint32_t k = lane_k(T.e0, c);
+					en_nx[c] = global_of(o.ent)[k];
+					key_nx[c] = global_of(o.keys)[k];
+				}
+			}
 			for (uint32_t e0 = T.e0; e0 < T.e1; e0 += kLaneEntries * G) {
 				// slot chunks that hold any entry (wave-uniform); the
 				// loads of empty chunks are skipped
@@ -282,17 +300,39 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				float sgn[kLaneEntries];
 				float4 acc[kLaneEntries];
 				const uint32_t bl = bvalid ? (uint32_t)b : 0u;
+				uint2 en_c[kLaneEntries];
+#pragma unroll
+				for (int c = 0; c < kLaneEntries; c++) {
+					if constexpr (kAhead == 2) {
+						en_c[c] = en_nx[c];
+						outc[c] = key_nx[c];
+					} else {
+						const uint32_t k = lane_k(e0, c);
+						en_c[c] = global_of(o.ent)[k];
+						outc[c] = global_of(o.keys)[k];
+					}
+				}
+				if constexpr (kAhead == 2) {
+					const uint32_t en0 = e0 + kLaneEntries * G;
+					if (en0 < T.e1) {
+#pragma unroll
+						for (int c = 0; c < kLaneEntries; c++) {
+							const uint32_t k = lane_k(en0, c);
+							en_nx[c] = global_of(o.ent)[k];
+							key_nx[c] = global_of(o.keys)[k];
+						}
+					}
+				}
 #pragma unroll
 				for (int c = 0; c < kLaneEntries; c++) {
 					const uint32_t e = e0 + (uint32_t)(c * G + slot);
 					vl[c] = e < T.e1 && bvalid;
-					const uint32_t k = I.n_base + (e < T.e1 ? e : T.e0);
-					const uint2 en = global_of(o.ent)[k];
+					const uint32_t k = lane_k(e0, c);
+					const uint2 en = en_c[c];
 					// element column: compact index, or lane-list
 					// position for expanded element rows
 					colb[c] = (I.expanded ? k - I.n_base : en.y & kMask) * (uint32_t)sizeof(VT);
 					xob[c] = (en.x * (uint32_t)B + bl) * 4u;
-					outc[c] = global_of(o.keys)[k];
 					sgn[c] = sign_of(1.f, en.y);
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 				}
