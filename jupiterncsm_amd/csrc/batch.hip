@@ -1698,18 +1698,31 @@ __global__ __launch_bounds__(256) void col_dot_partial(const double *__restrict_
 	}
 }
 
-// out[j][b] = sum_c partial[j][c][b] in a fixed order
-__global__ void col_dot_reduce(const double *__restrict__ partial, int chunks, int B,
-			       const int64_t *kp, int self, double *__restrict__ out)
+// out[j][b] = sum_c partial[j][c][b] in a fixed order: block j, thread
+// (g, b) of the 256 / B chunk groups adds chunks g, g + 256 / B, ..., then
+// the groups' sums are added in LDS (B <= kMaxVectors = 256)
+__global__ __launch_bounds__(256) void col_dot_reduce(const double *__restrict__ partial, int chunks,
+						     int B, const int64_t *kp, int self,
+						     double *__restrict__ out)
 {
+	__shared__ double sh[256];
 	const int j = blockIdx.x;
 	if (self ? j > 0 : j > *kp)
 		return;
-	for (int b = threadIdx.x; b < B; b += blockDim.x) {
-		double t = 0;
-		for (int c = 0; c < chunks; c++)
-			t += partial[((int64_t)j * chunks + c) * B + b];
-		out[(int64_t)j * B + b] = t;
+	const int G = 256 / B;
+	const int t = threadIdx.x;
+	const int g = t / B, b = t - g * B;
+	double v = 0;
+	if (g < G)
+		for (int c = g; c < chunks; c += G)
+			v += partial[((int64_t)j * chunks + c) * B + b];
+	sh[t] = v;
+	__syncthreads();
+	if (t < B) {
+		double s = 0;
+		for (int q = 0; q < G; q++)
+			s += sh[q * B + t];
+		out[(int64_t)j * B + t] = s;
 	}
 }
 
