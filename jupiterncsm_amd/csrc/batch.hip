@@ -295,9 +295,12 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 				const int nc = (int)min((uint32_t)kLaneEntries, (T.e1 - e0 + G - 1) / (uint32_t)G);
 				bool vl[kLaneEntries];
 				// byte offsets of the lane's element (from the element
-				// row) and of its 4 input values (from the input row)
+				// row) and of its 4 input values (from the input row); the
+				// lane's tile offset within a row (output column x pitch +
+				// first vector) and its phase bit
 				uint32_t colb[kLaneEntries], xob[kLaneEntries], outc[kLaneEntries];
-				float sgn[kLaneEntries];
+				int lidx[kLaneEntries];
+				uint32_t lsb[kLaneEntries];
 				float4 acc[kLaneEntries];
 				const uint32_t bl = bvalid ? (uint32_t)b : 0u;
 				uint2 en_c[kLaneEntries];
@@ -333,7 +336,8 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					// position for expanded element rows
 					colb[c] = (I.expanded ? k - I.n_base : en.y & kMask) * (uint32_t)sizeof(VT);
 					xob[c] = (en.x * (uint32_t)B + bl) * 4u;
-					sgn[c] = sign_of(1.f, en.y);
+					lidx[c] = (int)outc[c] * pitch + q4;
+					lsb[c] = en.y & 0x80000000u;
 					acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 				}
 				auto rows_pass = [&](auto full) {
@@ -368,15 +372,14 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 					}
 				};
 				int cur = -1;
+				// the sums carry both phases (row entry and lane entry), so
+				// a flush scales them by the vectors' force scales only
 				auto flush = [&]() {
-					const int r = cur - w.po_begin;
+					const int rb = (cur - w.po_begin) * nn * pitch;
 #pragma unroll
 					for (int c = 0; c < kLaneEntries; c++)
-						if (vl[c]) {
-							const float4 s4 = make_float4(sgn[c] * sc.x, sgn[c] * sc.y,
-										      sgn[c] * sc.z, sgn[c] * sc.w);
-							add4_lds(&tile[(r * nn + (int)outc[c]) * pitch + q4], s4, acc[c]);
-						}
+						if (vl[c])
+							add4_lds(&tile[rb + lidx[c]], sc, acc[c]);
 				};
 				// products of entry t from buffer buf, after a flush when
 				// its row differs from the current one
@@ -391,10 +394,11 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 							acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
 						cur = row;
 					}
-					const uint32_t py = __builtin_amdgcn_readlane(pl.y, t);
+					const uint32_t ps = __builtin_amdgcn_readlane(pl.y, t) & 0x80000000u;
 #pragma unroll
 					for (int c = 0; c < kLaneEntries; c++)
-						fma4(acc[c], sign_of(va[k][c], py), xa[k][c]);
+						fma4(acc[c], __int_as_float(__float_as_int(va[k][c]) ^ (int)(ps ^ lsb[c])),
+						     xa[k][c]);
 				};
 				using B0 = std::integral_constant<int, 0>;
 				using B1 = std::integral_constant<int, 1>;

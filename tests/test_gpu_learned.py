@@ -37,6 +37,26 @@ def test_basis_kernels_match_numpy():
     assert np.abs(s.cpu().numpy() - ref).max() <= 1e-12
 
 
+def test_basis_transform_few_and_many_outputs():
+    """out = C U on the device: 20 outputs (one pass over U per output) and
+    the first 8 of them (one pass over U for all outputs) against numpy in
+    FP64, and bit-equal to each other (the same FP64 sums in the same order,
+    rounded once to FP32)."""
+    print("case basis transform", flush=True)
+    torch = torch_dev()
+    rng = np.random.default_rng(3)
+    n, D = 52, 10007
+    U = rng.standard_normal((n, D)).astype(np.float32)
+    C = rng.standard_normal((20, n))
+    Ud = torch.as_tensor(U, device="cuda")
+    many = bacchus.basis_transform(Ud, C).cpu().numpy()
+    few = bacchus.basis_transform(Ud, C[:8]).cpu().numpy()
+    ref = C @ U.astype(np.float64)
+    tol = 2.0 ** -23 * np.abs(ref).max()
+    assert np.abs(many - ref).max() <= tol
+    assert np.array_equal(few, many[:8])
+
+
 def test_learned_start_beats_random_start():
     """2+2 Nmax 6 family: 64 batched training solves, 16-vector basis, a
     small MLP trained on the GPU; on held-out points the lowest eigenvalue
