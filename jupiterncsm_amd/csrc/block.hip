@@ -37,13 +37,14 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t z)
 }
 
 // partial[(j * chunks + c) * b * b + p * b + q] = sum over the rows r of
-// chunk c of Q_j[r][p] * W[r][q]; Q holds nblk blocks [D][b].  Thread
+// chunk c of Q_j[r][p] * W[r][q]; Q holds nq blocks [D][b], and a grid row
+// j = nq (grid.y = nq + 1) takes W itself in place of Q_j.  Thread
 // (g, q) of the block's 256 / b row groups walks rows g, g + 256 / b, ...
 // of the chunk with the b sums of column q in registers (its W loads and
 // the row's Q values are contiguous across the wave), then the row groups'
 // sums are added in LDS in a fixed order.
 __global__ __launch_bounds__(256) void block_inner(const float *__restrict__ Q,
-						   const float *__restrict__ W, int64_t D, int b,
+						   const float *__restrict__ W, int64_t D, int b, int nq,
 						   double *__restrict__ partial)
 {
 	__shared__ double sh[256 * kMaxBlock];
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256) void block_inner(const float *__restrict__ Q,
 	const int g = t / b, q = t - g * b;
 	const int64_t per = (D + gridDim.x - 1) / gridDim.x;
 	const int64_t r0 = per * blockIdx.x, r1 = min(D, r0 + per);
-	const float *Qj = Q + (int64_t)j * D * b;
+	const float *Qj = j < nq ? Q + (int64_t)j * D * b : W;
 	double s[kMaxBlock];
 #pragma unroll
 	for (int p = 0; p < kMaxBlock; p++)
@@ -247,12 +248,14 @@ struct Block {
 	}
 
 	// C[(j b + p) b + q] = Q_j[:, p] . V[:, q] for the blocks j0 <= j < j0 + nblk
-	int inner(const float *Qs, int nblk, const float *V, std::vector<double> &C)
+	// with_self: one more block of results, V^T V, after the nblk ones
+	int inner(const float *Qs, int nblk, const float *V, std::vector<double> &C, bool with_self = false)
 	{
 		const int bb = b * b;
-		hipLaunchKernelGGL(block_inner, dim3(kInnerChunks, (unsigned)nblk), dim3(256), 0, nullptr, Qs, V,
-				   D, b, dpart);
-		const int n = nblk * bb;
+		const int rows = nblk + (with_self ? 1 : 0);
+		hipLaunchKernelGGL(block_inner, dim3(kInnerChunks, (unsigned)rows), dim3(256), 0, nullptr, Qs, V,
+				   D, b, nblk, dpart);
+		const int n = rows * bb;
 		hipLaunchKernelGGL(block_inner_reduce, dim3((unsigned)n), dim3(256), 0, nullptr, dpart,
 				   kInnerChunks, bb, dsum);
 		C.resize((size_t)n);
@@ -391,16 +394,24 @@ struct Block {
 	int first_round(int nblk, float *V, float *out, std::vector<double> &coef,
 			std::vector<double> &R, uint64_t seed, double *cond)
 	{
-		std::vector<double> G0;
-		if (int rc = inner(V, 1, V, G0))
+		// Q^T V for the stored blocks and V^T V in one launch
+		std::vector<double> C;
+		if (int rc = inner(Q, nblk, V, C, true))
 			return rc;
+		const size_t bb = (size_t)b * b;
+		const std::vector<double> G0(C.end() - bb, C.end());
+		C.resize(C.size() - bb);
 		double scale = 0;
 		for (int k = 0; k < b; k++)
 			scale = std::max(scale, G0[(size_t)k * b + k]);
 		const double floor = 1e-24 * scale;
-		coef.assign((size_t)nblk * b * b, 0.0);
-		if (int rc = project_out(nblk, V, &coef))
-			return rc;
+		coef = C;
+		if (nblk > 0) {
+			for (double &c : C)
+				c = -c;
+			if (int rc = combine<float>(Q, nblk, C, b, V, V, 0))
+				return rc;
+		}
 		// with no stored blocks V is unchanged and G0 is its Gram matrix
 		return svqb(nblk, V, out, R, floor, seed, cond, nblk == 0 ? &G0 : nullptr);
 	}
