@@ -295,7 +295,14 @@ def main():
         bt = bacchus.Batch([op32], [args.block])
         bt.set_scales(0, np.ones((args.block, 3)))
         tuned = bt.autotune(3)
+        bst = bt.stats()
         block_mv = {"vectors": args.block, "ms": round(tuned["ms"], 4),
+                    # its algorithmic bytes (elements and lists once, the 8
+                    # vectors in and out) over the launch time
+                    "algorithmic_gb_s": round(bst["algorithmic_bytes"] / (tuned["ms"] * 1e-3) / 1e9, 1),
+                    "hbm_frac": round(bst["algorithmic_bytes"] / (tuned["ms"] * 1e-3) / 1e9
+                                      / HBM_PEAK_GBS, 4),
+                    "tflop_s": round(bst["flops"] / (tuned["ms"] * 1e-3) / 1e12, 2),
                     "mfma": bt.mfma_stats()}
         log(f"block matvec ({args.block} vectors) autotuned: {tuned['ms']:.2f} ms, "
             f"matrix cores {block_mv['mfma']}")
