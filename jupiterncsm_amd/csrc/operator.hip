@@ -239,13 +239,15 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(
 					const uint32_t e = e0 + 128 * c + 2 * lane;
 					vl[c] = e < T.e1;
 					const uint32_t k = I.n_base + (vl[c] ? e : T.e0);
-					const uint2 ea = ent[k], eb = ent[k + 1];
+					// k is even (paired copies start at even entries)
+					const uint4 eab = *reinterpret_cast<const uint4 *>(ent + k);
+					const uint2 kab = *reinterpret_cast<const uint2 *>(keys + k);
 					col[c] = k - I.n_base;
-					xin[c] = ea.x;
-					outa[c] = keys[k];
-					outb[c] = keys[k + 1];
-					sga[c] = apply_sign(sc, ea.y);
-					sgb[c] = apply_sign(sc, eb.y);
+					xin[c] = eab.x;
+					outa[c] = kab.x;
+					outb[c] = kab.y;
+					sga[c] = apply_sign(sc, eab.y);
+					sgb[c] = apply_sign(sc, eab.w);
 					acca[c] = accb[c] = 0;
 				}
 				auto pairs_pass = [&](auto live) {
@@ -1343,13 +1345,21 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	{
 		size_t total = entries.size();
 		for (const LaneJob &j : lane_jobs)
-			total += j.e.size();
+			total += j.e.size() + 1;
 		if (total > 0xFFFFFFFFull)
 			return fail(BACCHUS_FAIL("index lists exceed 2^32 entries"));
 		entries.reserve(total);
 		keys.reserve(total);
 		pad_entry.reserve(total);
 		for (LaneJob &j : lane_jobs) {
+			// paired copies start at an even entry, so a pair's two
+			// entries are one aligned 16-byte load and their keys one
+			// 8-byte load
+			if (pair_lanes && expanded[j.l] && entries.size() % 2) {
+				entries.push_back(make_uint2(0u, 0u));
+				keys.push_back(0u);
+				pad_entry.push_back(1);
+			}
 			const int64_t start = (int64_t)entries.size();
 			(j.dir == 0 ? elem_by_out : elem_by_in)[j.l] = start;
 			(j.dir == 0 ? elen_by_out : elen_by_in)[j.l] = (int64_t)j.e.size();
