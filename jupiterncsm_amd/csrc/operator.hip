@@ -1666,14 +1666,14 @@ static int operator_create(const bacchus_hip_problem_t *p, int precision,
 	pp.xcds = kXcds;
 	pp.skip_np = op->np_path;
 	if (const char *xe = getenv("BACCHUS_HIP_XCDS"))
-		pp.xcds = atoi(xe);
+		pp.xcds = std::max(1, std::min(64, atoi(xe)));
 	// tiles grouped by the input blocks they read, eight groups per XCD
 	// (3+3 Nmax 6 FP64 matvec: 4.3 ms grouped by output block, 3.80 ms
 	// with one input group per XCD, 3.39 ms with 64; 16, 32 and 48 groups
 	// 3.62, 3.48, 3.45 ms; 96 and more 3.51 ms)
 	pp.in_groups = 8 * kXcds;
 	if (const char *ge = getenv("BACCHUS_HIP_IN_GROUPS"))
-		pp.in_groups = atoi(ge);
+		pp.in_groups = std::max(1, std::min(4096, atoi(ge)));
 	// LDS doubles per tile (measurement; the kernel's LDS follows the plan)
 	if (const char *te = getenv("BACCHUS_HIP_TILE_ELEMS"))
 		pp.tile_elems = std::max<int64_t>(1024, std::min<int64_t>(16384, atoll(te)));
