@@ -237,7 +237,9 @@ __device__ void batch_tile(const BatchOp &o, const TileWork &w, int b0, int lbt,
 	const int b = b0 + q4;
 	const bool bvalid = b < B;    // B is a multiple of 4: all 4 or none
 	const VT *V = (const VT *)o.V;
-	const float *xs = w.space ? o.xT : o.x;
+	// uniform, so the input-row addresses of the NP loop stay scalar
+	const float *xs = reinterpret_cast<const float *>(
+		uniform_ptr(reinterpret_cast<const char *>(w.space ? o.xT : o.x)));
 	// descriptor words of the current task, its instruction, and the next
 	// task (the next task's instruction is loaded when this one starts)
 	int32_t ti = w.t_begin + wave;
