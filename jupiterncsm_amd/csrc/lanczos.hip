@@ -46,22 +46,44 @@ __device__ double block_sum(double v, double *sh)
 }
 
 // partial[j * nblk + blk] = sum over this block's slice of U_j . w,
-// j = 0..nvec-1 (nvec = 1 gives a plain dot product)
+// j = 0..nvec-1 (nvec = 1 gives a plain dot product).  Grid (nblk, vector
+// groups of kDotVecs): a thread reads w[i] once for the group's vectors and
+// keeps their sums in registers; each sum is reduced over the block in a
+// fixed order (wave shuffles, then the waves in order).
+constexpr int kDotVecs = 8;
 __global__ __launch_bounds__(kThreads) void multi_dot_partial(
 	const double *__restrict__ U, int64_t ld, int nvec,
 	const double *__restrict__ w, int64_t n, double *__restrict__ partial)
 {
-	__shared__ double sh[kThreads / 64];
+	__shared__ double sh[kDotVecs][kThreads / 64];
+	const int j0 = blockIdx.y * kDotVecs;
+	const int nv = min(kDotVecs, nvec - j0);
 	const int64_t per = (n + gridDim.x - 1) / gridDim.x;
 	const int64_t b = per * blockIdx.x, e = min(n, b + per);
-	for (int j = 0; j < nvec; j++) {
-		const double *u = U + ld * j;
-		double s = 0;
-		for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x)
-			s += u[i] * w[i];
-		s = block_sum(s, sh);
-		if (threadIdx.x == 0)
-			partial[(int64_t)j * gridDim.x + blockIdx.x] = s;
+	double s[kDotVecs];
+#pragma unroll
+	for (int q = 0; q < kDotVecs; q++)
+		s[q] = 0.0;
+	for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+		const double wi = w[i];
+#pragma unroll
+		for (int q = 0; q < kDotVecs; q++)
+			if (q < nv)
+				s[q] += U[ld * (j0 + q) + i] * wi;
+	}
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+	for (int q = 0; q < kDotVecs; q++) {
+		const double v = wave_sum(s[q]);
+		if (lane == 0)
+			sh[q][wid] = v;
+	}
+	__syncthreads();
+	if ((int)threadIdx.x < nv) {
+		double t = 0;
+		for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+			t += sh[threadIdx.x][k];
+		partial[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
 	}
 }
 
@@ -105,6 +127,7 @@ __global__ void subtract_combination(double *__restrict__ w,
 	if (i >= n)
 		return;
 	double v = w[i];
+#pragma unroll 4
 	for (int j = 0; j < nvec; j++)
 		v -= c[j] * U[ld * j + i];
 	w[i] = v;
@@ -236,7 +259,7 @@ struct Lanczos {
 
 	int dot(const double *a, const double *b, double *out)
 	{
-		hipLaunchKernelGGL(multi_dot_partial, dim3(nblk), dim3(kThreads), 0, s, a, D, 1,
+		hipLaunchKernelGGL(multi_dot_partial, dim3(nblk, 1), dim3(kThreads), 0, s, a, D, 1,
 				   b, D, partial);
 		hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(kThreads), 0, s, partial,
 				   nblk, out);
@@ -244,8 +267,9 @@ struct Lanczos {
 	}
 	int multi_dot(const double *U, int nvec, const double *w, double *out)
 	{
-		hipLaunchKernelGGL(multi_dot_partial, dim3(nblk), dim3(kThreads), 0, s, U, D,
-				   nvec, w, D, partial);
+		hipLaunchKernelGGL(multi_dot_partial,
+				   dim3(nblk, (unsigned)((nvec + kDotVecs - 1) / kDotVecs)), dim3(kThreads),
+				   0, s, U, D, nvec, w, D, partial);
 		hipLaunchKernelGGL(reduce_partials, dim3(nvec), dim3(kThreads), 0, s, partial,
 				   nblk, out);
 		return 0;
