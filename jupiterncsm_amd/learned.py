@@ -331,13 +331,15 @@ class ProjectedFamily:
     def ritz(self, c1, c2, k):
         """Lowest k Ritz values and coefficient vectors (rank, k) over the
         basis rows, normalized in the N inner product."""
-        M = self.parts[0] + c1 * self.parts[1] + c2 * self.parts[2]
-        # N = L L^T; eigen of L^-1 M L^-T (L^-1 is the same for every point)
-        Li = getattr(self, "_gram_cholesky_inverse", None)
-        if Li is None:
+        # N = L L^T; eigen of L^-1 M L^-T with M = P_1 + c1 P_2 + c2 P_3, so
+        # the three L^-1 P_k L^-T are formed once for every point
+        cache = getattr(self, "_whitened_parts", None)
+        if cache is None:
             Li = np.linalg.inv(np.linalg.cholesky(self.gram))
-            self._gram_cholesky_inverse = Li
-        A = Li @ M @ Li.T
+            cache = (Li, [Li @ P @ Li.T for P in self.parts])
+            self._whitened_parts = cache
+        Li, (A1, A2, A3) = cache
+        A = A1 + c1 * A2 + c2 * A3
         w, Z = np.linalg.eigh(0.5 * (A + A.T))
         return w[:k], Li.T @ Z[:, :k]
 
