@@ -164,6 +164,36 @@ def test_c3_full_size_matches_oracle():
     assert rel_err(h32, ref) <= 2e-6
 
 
+@pytest.mark.parametrize("name", ["he4_nmax6", "li6_nmax2_3b"])
+def test_element_row_major_np_kernel_matches_oracle(name):
+    """The element-row-major NP kernel (np.hip, selected by BACCHUS_HIP_NP=1
+    when the operator is built; off by default, where the tile kernel runs
+    the NP instructions): FP64 against the oracle to 1e-12 of max|y| with
+    force scales other than 1, FP32 storage within 2e-6."""
+    print(f"case element-row-major np {name}", flush=True)
+    inp = build(SYSTEMS[name] if name in SYSTEMS else CONFIGS[name])
+    p = O.Problem.from_inputs(inp)
+    x = np.random.default_rng(21).standard_normal(inp.dimension)
+    sc = [1.0, 0.9, 1.1]
+    saved = os.environ.get("BACCHUS_HIP_NP")
+    os.environ["BACCHUS_HIP_NP"] = "1"
+    try:
+        op = bacchus.Operator.from_inputs(inp)
+        op32 = bacchus.Operator.from_inputs(inp, precision=bacchus.FP32)
+    finally:
+        if saved is None:
+            del os.environ["BACCHUS_HIP_NP"]
+        else:
+            os.environ["BACCHUS_HIP_NP"] = saved
+    with op, op32:
+        op.set_scales(*sc)
+        op32.set_scales(*sc)
+        y, y32 = gpu_matvec(op, x), gpu_matvec(op32, x)
+    y_o = O.matvec(p, x, scales=sc)
+    assert rel_err(y, y_o) <= FP64_TOL
+    assert rel_err(y32, y_o) <= 2e-6
+
+
 @pytest.mark.parametrize("scale", SK.SCALES)
 @pytest.mark.parametrize("case", ["N", "P narrow", "P wide", "NP"])
 def test_tiny_element_skip(case, scale):
