@@ -66,6 +66,30 @@ def test_batched_matvec_uneven_operators_match_oracle(precision, tol):
     assert min(errs) > 0.0
 
 
+@pytest.mark.parametrize("env", [{"BACCHUS_HIP_BATCH_GROUPS": "16"},
+                                 {"BACCHUS_HIP_BATCH_ORDER": "1"},
+                                 {"BACCHUS_HIP_BATCH_GROUPS": "1", "BACCHUS_HIP_BATCH_MFMA": "0"}])
+def test_batched_queue_switches_keep_results(env):
+    """The batched plan's switches (input groups of the work queues, the
+    order of a queue's items, the matrix-core path off) change the schedule,
+    not the results: FP32 against the oracle within 2e-6."""
+    import os
+    print(f"case batch switches {env}", flush=True)
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        errs = batch_vs_oracle([SYSTEMS["he4_nmax4"], SYSTEMS["he4_nmax6"], SMALL_3B], [3, 5, 2],
+                               bacchus.FP32, seed=4)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    print("errors", errs, flush=True)
+    assert max(errs) <= 2e-6
+
+
 @pytest.mark.parametrize("counts,vectors", [([1, 1], 0), ([64, 17], 0), ([33, 7], 16)])
 def test_batched_matvec_vector_widths(counts, vectors):
     """One vector, a full 64-wide tile and a tile width cap."""

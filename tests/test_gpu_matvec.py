@@ -164,6 +164,33 @@ def test_c3_full_size_matches_oracle():
     assert rel_err(h32, ref) <= 2e-6
 
 
+@pytest.mark.parametrize("env", [{"BACCHUS_HIP_IN_GROUPS": "1"}, {"BACCHUS_HIP_XCDS": "1"},
+                                 {"BACCHUS_HIP_TILE_ORDER": "0"}, {"BACCHUS_HIP_EXPAND": "0"},
+                                 {"BACCHUS_HIP_EXPAND": "1000:1000"},
+                                 {"BACCHUS_HIP_TILE_ELEMS": "8192", "BACCHUS_HIP_TILE_MULTS": "4000"}])
+def test_plan_switches_keep_results(env):
+    """The single-vector plan's switches (input groups, XCD grouping, tile
+    order, which lane lists get expanded element rows, tile size and work)
+    change the schedule and layout, not the results: FP64 against the oracle
+    to 1e-12 of max|y| on a 3-body system."""
+    print(f"case plan switches {env}", flush=True)
+    inp = build(CONFIGS["li6_nmax2_3b"])
+    x = np.random.default_rng(23).standard_normal(inp.dimension)
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        op = bacchus.Operator.from_inputs(inp)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    with op:
+        y = gpu_matvec(op, x)
+    assert rel_err(y, O.matvec(O.Problem.from_inputs(inp), x)) <= FP64_TOL
+
+
 @pytest.mark.parametrize("name", ["he4_nmax6", "li6_nmax2_3b"])
 def test_element_row_major_np_kernel_matches_oracle(name):
     """The element-row-major NP kernel (np.hip, selected by BACCHUS_HIP_NP=1
