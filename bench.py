@@ -5,31 +5,34 @@ with seeded Gaussian 2-body (sigma 1 MeV) and 3-body (sigma 0.1 MeV)
 elements and the one-body term 20(N+1.5) MeV: D = 197,822.
 
 A step is one solve of the lowest 5 eigenvalues to 1e-6 relative accuracy
-against FP64 reference eigenvalues (the FP64 GPU path run to 300 Lanczos
-iterations with full reorthogonalization) with the fast path:
+with the fast path; the reference eigenvalues are the oracle's 300-iteration
+FP64 Lanczos from a seeded start (tests/golden/lanczos300_<workload>.npz),
+which the FP64 GPU path run the same way is checked against:
 
 - FP32-stored matrix elements with FP32 products and accumulation;
 - block Lanczos, 8 vectors per block, each block step one batched launch,
   FP64 inner products and block tridiagonal solve, stopping on its own
-  residual bound |r|^2 / gap <= 1e-6 |theta|;
-- started from the learned stored basis: 64 FP32 vectors spanning band
-  eigenvectors of batched solves of the family H(c1, c2) = H1 + c1 V2 +
-  c2 V3, onto which H1, V2, V3 are projected once; the start block is the
-  Ritz vectors of H(c1, c2) in that span (training and projections
-  untimed).
+  residual bound |r_k|^2 / gap_k <= 1e-6 |theta_k| (gap_k corrected by the
+  other Ritz values' residuals);
+- started from a projected family: the band Ritz vectors of batched solves
+  of the family H(c1, c2) = H1 + c1 V2 + c2 V3 (--projected-basis of them,
+  orthonormalized to their numerical rank) with H1, V2, V3 projected onto
+  them once; the start block is the Ritz vectors of H(c1, c2) in that span
+  (solves and projections untimed, reported as setup_seconds and amortized
+  per solve).
 
-The plain FP64 GPU path (single-vector Lanczos, random start) is timed to
-the target criterion beside it, and so are two starts the MLPs
-(2-256x3-64, GELU, BF16 weights) predict over the same basis: the
-single-vector FP32 Lanczos from one network's start vector (mlp_start), and
-the same block Lanczos from the block of --block networks, each predicting
-the projection of its own fixed vector onto the band (mlp_block_start; its
-own residual bound, and also the target criterion the FP64 path stops on).
-The fast path and the MLP block start are also run at --heldout further
-(c1, c2) points, each checked against its own FP64 reference.  The
-FP32- and BF16-stored operators go through the reference's own solve (300
-iterations, same start) as well, and the line states their error per
-eigenvalue (relative_errors_by_storage).
+The plain FP64 GPU path (single-vector Lanczos, seeded start, full
+reorthogonalization) is timed to the target criterion beside it, and so are
+two starts the MLPs (2-256x3-64, GELU, BF16 weights) predict over a stored
+basis of 64 vectors: the single-vector FP32 Lanczos from one network's
+start vector (mlp_start), and the same block Lanczos from the block of
+--block networks, each predicting the projection of its own fixed vector
+onto the band (mlp_block_start; its own residual bound, with the target
+criterion reported only as a check).  The fast path and the MLP block start
+are also run at --heldout further (c1, c2) points, each against its own FP64
+reference.  The FP32- and BF16-stored operators (and a mixed store) go
+through the reference's own solve (300 iterations, same start) as well, and
+the line states their error per eigenvalue (relative_errors_by_storage).
 
 Other modes (one JSON line each):
   --batched   64 Hamiltonians H(c1, c2) over the 2+2 systems at Nmax 4, 6,
@@ -38,11 +41,15 @@ Other modes (one JSON line each):
   --learned   the learned start of the 3+3 Nmax 6 ground state: 512 batched
               training solves, 64 held-out points, iterations and time to
               1e-6 of the lowest eigenvalue against a random start.
+The default line carries both under "configs", with C1/C2.
 
-The line also carries the roofline of the dominant kernel (the FP64 matvec,
-HBM-bound: algorithmic bytes = 8 sum|V| + 12 sum|L| + 16 D per launch over
-its HIP-event time on the launch stream) and the CPU oracle timed on the
-host's cores on a bounded sample.
+The line also carries the roofline of the metric's plain matvec (the FP64
+tile kernel, HBM-bound: algorithmic bytes = 8 sum|V| + 12 sum|L| + 16 D per
+launch over its HIP-event time on the launch stream, the HBM traffic from
+the rocprofv3 summary in profiles/), the same figures for the fast path's
+8-vector batched matvec (block_matvec), and the CPU baseline: the oracle's
+FP64 Lanczos of the workload on the host's cores for the iteration count the
+FP64 GPU path took.
 """
 import argparse
 import json
