@@ -74,12 +74,12 @@ def parse():
     ap.add_argument("--eigenvalues", type=int, default=5)
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--matvecs", type=int, default=20, help="timed matvecs for the roofline")
-    ap.add_argument("--cpu-matvecs", type=int, default=60)
     ap.add_argument("--train-points", type=int, default=128,
                     help="family solves the learned start is trained on")
     ap.add_argument("--train-iterations", type=int, default=80)
     ap.add_argument("--no-learned", action="store_true",
-                    help="fast path from a random start instead of the learned one")
+                    help="no family solves, MLPs or projected start: the fast path is the "
+                         "single-vector FP32 Lanczos from the seeded start")
     ap.add_argument("--start-mix", type=float, default=1e-4,
                     help="relative size of the random component mixed into the MLP start")
     ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
