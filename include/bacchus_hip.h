@@ -396,7 +396,9 @@ int bacchus_hip_basis_project_static(const float *U, int64_t n, int64_t D,
 int bacchus_hip_basis_combine(const float *U, int64_t n, int64_t D,
 			      const float *c, double *out, void *stream);
 /*   transform: out[j] = sum_i C[j][i] U_i for j < r (C host FP64 r x n;
- *              out device FP32 [r][D]) - e.g. an orthonormalized basis */
+ *              out device FP32 [r][D]) - e.g. an orthonormalized basis; C
+ *              goes through a device buffer the library keeps per device,
+ *              and calls are serialized; synchronous */
 int bacchus_hip_basis_transform(const float *U, int64_t n, int64_t D,
 				const double *C, int64_t r, float *out);
 
