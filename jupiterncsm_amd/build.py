@@ -60,14 +60,17 @@ def build_inputs(verbose=False):
 
 
 def build_hip(verbose=False):
+    """Compiles the HIP sources in parallel (one hipcc per source, at most
+    8 at a time) and links them."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(HERE, "libbacchus_hip.so")
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(CSRC, "_build", src.replace(".hip", ".o"))
-        os.makedirs(os.path.dirname(obj), exist_ok=True)
-        _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c",
-              os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+    os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
+    objs = [os.path.join(CSRC, "_build", src.replace(".hip", ".o")) for src in HIP_SOURCES]
+    cmds = [[HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, src),
+             "-o", obj] for src, obj in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out], verbose)
     return out
 
