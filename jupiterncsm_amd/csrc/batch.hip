@@ -1708,10 +1708,13 @@ __global__ __launch_bounds__(256) void col_dot_partial(const double *__restrict_
 
 // out[j][b] = sum_c partial[j][c][b] in a fixed order: block j, thread
 // (g, b) of the 256 / B chunk groups adds chunks g, g + 256 / B, ..., then
-// the groups' sums are added in LDS (B <= kMaxVectors = 256)
+// the groups' sums are added in LDS (B <= kMaxVectors = 256).  alpha (may
+// be null): the block j = k also stores (add = 0) or adds (add = 1) its
+// sums into alpha[k][b], the Lanczos coefficient of q_k
 __global__ __launch_bounds__(256) void col_dot_reduce(const double *__restrict__ partial, int chunks,
 						     int B, const int64_t *kp, int self,
-						     double *__restrict__ out)
+						     double *__restrict__ out,
+						     double *__restrict__ alpha = nullptr, int add = 0)
 {
 	__shared__ double sh[256];
 	const int j = blockIdx.x;
@@ -1731,6 +1734,10 @@ __global__ __launch_bounds__(256) void col_dot_reduce(const double *__restrict__
 		for (int q = 0; q < G; q++)
 			s += sh[q * B + t];
 		out[(int64_t)j * B + t] = s;
+		if (alpha && !self && j == *kp) {
+			double *a = alpha + (int64_t)j * B + t;
+			*a = add ? *a + s : s;
+		}
 	}
 }
 
@@ -1747,17 +1754,6 @@ __global__ void col_subtract(double *__restrict__ w, const double *__restrict__ 
 	for (int64_t j = 0; j <= k; j++)
 		v -= h[j * B + b] * Q[j * n + i];
 	w[i] = v;
-}
-
-// alpha[k][b] (=|+=) h[k][b]
-__global__ void record_alpha(double *alpha, const double *h, int B, const int64_t *kp, int add)
-{
-	const int b = threadIdx.x + blockIdx.x * blockDim.x;
-	if (b >= B)
-		return;
-	const int64_t k = *kp;
-	const double v = h[k * B + b];
-	alpha[k * B + b] = add ? alpha[k * B + b] + v : v;
 }
 
 // beta[k][b] = |w_b|, Q[k+1] = w / beta (zero after a breakdown), k += 1
@@ -1867,11 +1863,9 @@ struct BatchLanczos {
 		hipLaunchKernelGGL(col_dot_partial, dim3(kDotChunks, (unsigned)(K + 1)), dim3(256), 0, s,
 				   l.Q, l.w, l.D, l.B, l.k, 0, l.partial);
 		hipLaunchKernelGGL(col_dot_reduce, dim3((unsigned)(K + 1)), dim3(256), 0, s, l.partial,
-				   kDotChunks, l.B, l.k, 0, l.h);
+				   kDotChunks, l.B, l.k, 0, l.h, l.alpha, add);
 		hipLaunchKernelGGL(col_subtract, dim3(blocks_of(n)), dim3(256), 0, s, l.w, l.Q, l.h, n,
 				   l.B, l.k);
-		hipLaunchKernelGGL(record_alpha, dim3((l.B + 255) / 256), dim3(256), 0, s, l.alpha, l.h,
-				   l.B, l.k, add);
 	}
 	// one iteration of every instance: w = H q_k, CGS2 against q_0..q_k,
 	// beta, q_{k+1}
