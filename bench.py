@@ -10,7 +10,9 @@ FP64 Lanczos from a seeded start (tests/golden/lanczos300_<workload>.npz),
 which the FP64 GPU path run the same way is checked against:
 
 - FP32-stored matrix elements with FP32 products and accumulation;
-- block Lanczos, 8 vectors per block, each block step one batched launch,
+- block Lanczos, 5 vectors per block (one per eigenvalue; 8 took up to three
+  block steps at held-out points where 5 takes one), each block step one
+  batched launch,
   FP64 inner products and block tridiagonal solve, stopping on its own
   residual bound |r_k|^2 / gap_k <= 1e-6 |theta_k| (gap_k corrected by the
   other Ritz values' residuals);
@@ -47,7 +49,7 @@ The line also carries the roofline of the metric's plain matvec (the FP64
 tile kernel, HBM-bound: algorithmic bytes = 8 sum|V| + 12 sum|L| + 16 D per
 launch over its HIP-event time on the launch stream, the HBM traffic from
 the rocprofv3 summary in profiles/), the same figures for the fast path's
-8-vector batched matvec (block_matvec), and the CPU baseline: the oracle's
+batched matvec of one block (block_matvec), and the CPU baseline: the oracle's
 FP64 Lanczos of the workload on the host's cores for the iteration count the
 FP64 GPU path took.
 """
@@ -82,7 +84,8 @@ def parse():
                          "single-vector FP32 Lanczos from the seeded start")
     ap.add_argument("--start-mix", type=float, default=1e-4,
                     help="relative size of the random component mixed into the MLP start")
-    ap.add_argument("--block", type=int, default=8, help="vectors per block of the fast path")
+    ap.add_argument("--block", type=int, default=5,
+                    help="vectors per block of the fast path (at least --eigenvalues)")
     ap.add_argument("--projected-basis", type=int, default=128,
                     help="stored band Ritz vectors the projected start's basis is made of")
     ap.add_argument("--block-mlp-steps", type=int, default=8000,
@@ -346,15 +349,17 @@ def main():
                                 tolerance=args.tolerance)
 
     def mlp_block_runs(c1, c2, ref_ev):
-        """(seconds, blocks, max relative error) of the block solve from the
-        MLP start block: on its residual bound, then on the target criterion."""
+        """(seconds, blocks, max relative error, converged) of the block solve
+        from the MLP start block: on its residual bound, then on the target
+        criterion (converged: the criterion, not --max-blocks, stopped it)."""
         out = []
         for tg in (None, ref_ev):
             f = lambda: block_solve(c1, c2, mblock, tg)  # noqa: E731
             timed(f)
             dt, r = timed(f)
             out.append((dt, r.blocks,
-                        float((np.abs(r.eigenvalues - ref_ev) / np.abs(ref_ev)).max())))
+                        float((np.abs(r.eigenvalues - ref_ev) / np.abs(ref_ev)).max()),
+                        bool(r.converged)))
         return out
 
     def rel(ev):
@@ -377,8 +382,9 @@ def main():
 
     mlp_block = None
     if mblock is not None:
-        (tr, br, er), (tt, btg, et) = mlp_block_runs(1.0, 1.0, targets)
+        (tr, br, er, cr), (tt, btg, et, _) = mlp_block_runs(1.0, 1.0, targets)
         mlp_block = {"seconds": round(tr, 4), "blocks": br, "max_relative_error": er,
+                     "converged": cr,
                      "speedup_vs_fp64_gpu": round(t64 / tr, 3),
                      # stops on the known answer: a convergence check, not a speed
                      "target_criterion_check": {"seconds": round(tt, 4), "blocks": btg,
@@ -445,9 +451,10 @@ def main():
                 "max_seconds": round(float(a[:, 3].max()), 4),
                 "max_blocks": int(a[:, 4].max()),
                 "max_relative_error": float(a[:, 5].max()),
-                "target_criterion_mean_seconds": round(float(a[:, 6].mean()), 4),
-                "target_criterion_max_blocks": int(a[:, 7].max()),
-                "target_criterion_max_relative_error": float(a[:, 8].max())}
+                "converged": int(a[:, 6].sum()),
+                "target_criterion_mean_seconds": round(float(a[:, 7].mean()), 4),
+                "target_criterion_max_blocks": int(a[:, 8].max()),
+                "target_criterion_max_relative_error": float(a[:, 9].max())}
         log(f"held-out: {heldout}")
 
     # --- CPU oracle on the host cores: the whole FP64 solve -------------
