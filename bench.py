@@ -324,6 +324,12 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t, r
 
+    def timed_median(fn, n=3):
+        """Median time of n runs (after the caller's warm-up) and the last
+        result: one stall of the host does not make a point's figure."""
+        runs = [timed(fn) for _ in range(n)]
+        return float(np.median([dt for dt, _ in runs])), runs[-1][1]
+
     def single(op, seed, start=None):
         kw = dict(num_eigenvalues=args.eigenvalues, reorthogonalization=bacchus.REORTH_FULL,
                   convergence=bacchus.CONVERGE_TARGETS, tolerance=args.tolerance,
@@ -356,7 +362,7 @@ def main():
         for tg in (None, ref_ev):
             f = lambda: block_solve(c1, c2, mblock, tg)  # noqa: E731
             timed(f)
-            dt, r = timed(f)
+            dt, r = timed_median(f)
             out.append((dt, r.blocks,
                         float((np.abs(r.eigenvalues - ref_ev) / np.abs(ref_ev)).max()),
                         bool(r.converged)))
@@ -427,7 +433,7 @@ def main():
                                         convergence=bacchus.CONVERGE_RESIDUALS,
                                         tolerance=args.tolerance)
             timed(solve_at)
-            dt, r = timed(solve_at)
+            dt, r = timed_median(solve_at)
             err = float((np.abs(r.eigenvalues - ref_h) / np.abs(ref_h)).max())
             row = [dt, r.blocks, err]
             if mblock is not None:
