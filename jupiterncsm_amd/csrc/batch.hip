@@ -1238,20 +1238,25 @@ int build_plan(bacchus_hip_batch_t *bt)
 	return upload_ops(bt);
 }
 
-int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
+// the launch in three parts: operator i's inputs (output cleared, input
+// transposed), the batched kernels of all operators, operator i's outputs
+// (transposed-space sums added); launch_batch runs them on one stream
+void launch_batch_in(bacchus_hip_batch_t *bt, size_t i, hipStream_t s)
 {
-	for (size_t i = 0; i < bt->ops.size(); i++) {
-		const bacchus_hip_operator_t *op = bt->ops[i];
-		const int64_t n = op->dimension * bt->Bp[i];
-		hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->h_ops[i].y, n, 0.f);
-		if (op->has_transposed) {
-			hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_yT[i], n, 0.f);
-			hipLaunchKernelGGL(transpose_blocks<float>,
-					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
-					   bt->h_ops[i].x, bt->d_xT[i], op->d_blk_off, op->d_blk_np,
-					   op->d_blk_nn, bt->Bp[i]);
-		}
+	const bacchus_hip_operator_t *op = bt->ops[i];
+	const int64_t n = op->dimension * bt->Bp[i];
+	hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->h_ops[i].y, n, 0.f);
+	if (op->has_transposed) {
+		hipLaunchKernelGGL(fill_float, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_yT[i], n, 0.f);
+		hipLaunchKernelGGL(transpose_blocks<float>,
+				   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
+				   bt->h_ops[i].x, bt->d_xT[i], op->d_blk_off, op->d_blk_np,
+				   op->d_blk_nn, bt->Bp[i]);
 	}
+}
+
+void launch_batch_core(bacchus_hip_batch_t *bt, hipStream_t s)
+{
 	if (bt->num_items) {
 		hipLaunchKernelGGL(reset_queue, dim3(1), dim3(64), 0, s, bt->d_next, bt->nq);
 		hipLaunchKernelGGL(bt->tuning.waves_per_simd == 4 ? batch_kernel_w4 : batch_kernel,
@@ -1262,14 +1267,25 @@ int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
 		hipLaunchKernelGGL(batch_mfma_kernel, dim3((unsigned)((bt->mfma_items + 3) / 4)), dim3(256),
 				   0, s, bt->d_ops, bt->d_mfma_units, bt->d_mfma_kg, bt->d_mfma_a,
 				   bt->d_mfma_items, (int)bt->mfma_items);
-	for (size_t i = 0; i < bt->ops.size(); i++) {
-		const bacchus_hip_operator_t *op = bt->ops[i];
-		if (op->has_transposed)
-			hipLaunchKernelGGL(add_transposed_blocks<float>,
-					   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
-					   bt->h_ops[i].y, bt->d_yT[i], op->d_blk_off, op->d_blk_np,
-					   op->d_blk_nn, bt->Bp[i]);
-	}
+}
+
+void launch_batch_out(bacchus_hip_batch_t *bt, size_t i, hipStream_t s)
+{
+	const bacchus_hip_operator_t *op = bt->ops[i];
+	if (op->has_transposed)
+		hipLaunchKernelGGL(add_transposed_blocks<float>,
+				   dim3((unsigned)op->block_np.size(), kTransposeSplits), dim3(256), 0, s,
+				   bt->h_ops[i].y, bt->d_yT[i], op->d_blk_off, op->d_blk_np,
+				   op->d_blk_nn, bt->Bp[i]);
+}
+
+int launch_batch(bacchus_hip_batch_t *bt, hipStream_t s)
+{
+	for (size_t i = 0; i < bt->ops.size(); i++)
+		launch_batch_in(bt, i, s);
+	launch_batch_core(bt, s);
+	for (size_t i = 0; i < bt->ops.size(); i++)
+		launch_batch_out(bt, i, s);
 	BACCHUS_HIP_CHECK(hipGetLastError());
 	return 0;
 }
@@ -1835,9 +1851,21 @@ struct BatchLanczos {
 	std::vector<LState> st;
 	hipStream_t s = nullptr;
 	int64_t K = 0;
+	// after the batched matvec each operator's vector work (independent of
+	// the others') runs on a stream of its own, forked from and joined back
+	// to s by events, so the operators' chains of small launches overlap
+	// (parallel branches when captured); all streams at the default priority
+	std::vector<hipStream_t> os;
+	// ev[0]: iteration start; ev[1]: matvec done; ev[2 + i], ev[2 + n + i]:
+	// operator i's inputs ready, its iteration done (n operators)
+	std::vector<hipEvent_t> ev;
 
 	~BatchLanczos()
 	{
+		for (auto e : ev)
+			(void)hipEventDestroy(e);
+		for (auto x : os)
+			(void)hipStreamDestroy(x);
 		for (auto &l : st) {
 			(void)hipFree(l.Q);
 			(void)hipFree(l.w);
@@ -1850,14 +1878,14 @@ struct BatchLanczos {
 		}
 	}
 
-	void norm_sq(LState &l, const double *u)
+	void norm_sq(LState &l, const double *u, hipStream_t s)
 	{
 		hipLaunchKernelGGL(col_dot_partial, dim3(kDotChunks, 1), dim3(256), 0, s, nullptr, u,
 				   l.D, l.B, l.k, 1, l.partial);
 		hipLaunchKernelGGL(col_dot_reduce, dim3(1), dim3(256), 0, s, l.partial, kDotChunks, l.B,
 				   l.k, 1, l.ss);
 	}
-	void project(LState &l, int add)
+	void project(LState &l, int add, hipStream_t s)
 	{
 		const int64_t n = l.D * l.B;
 		hipLaunchKernelGGL(col_dot_partial, dim3(kDotChunks, (unsigned)(K + 1)), dim3(256), 0, s,
@@ -1871,25 +1899,50 @@ struct BatchLanczos {
 	// beta, q_{k+1}
 	int iteration(bacchus_hip_batch_t *bt)
 	{
-		for (size_t i = 0; i < st.size(); i++) {
+		const bool fork = !os.empty();
+		const size_t n_ops = st.size();
+		if (fork)
+			BACCHUS_HIP_CHECK(hipEventRecord(ev[0], s));
+		for (size_t i = 0; i < n_ops; i++) {
 			LState &l = st[i];
 			const int Bp = bt->Bp[i];
-			hipLaunchKernelGGL(to_float_cols, dim3(blocks_of(l.D * Bp)), dim3(256), 0, s, l.Q, l.k,
+			hipStream_t x = s;
+			if (fork) {
+				BACCHUS_HIP_CHECK(hipStreamWaitEvent(os[i], ev[0], 0));
+				x = os[i];
+			}
+			hipLaunchKernelGGL(to_float_cols, dim3(blocks_of(l.D * Bp)), dim3(256), 0, x, l.Q, l.k,
 					   l.D, l.B, Bp, bt->d_x[i]);
+			launch_batch_in(bt, i, x);
+			if (fork) {
+				BACCHUS_HIP_CHECK(hipEventRecord(ev[2 + i], x));
+				BACCHUS_HIP_CHECK(hipStreamWaitEvent(s, ev[2 + i], 0));
+			}
 		}
-		if (int rc = launch_batch(bt, s))
-			return rc;
-		for (size_t i = 0; i < st.size(); i++) {
+		launch_batch_core(bt, s);
+		if (fork)
+			BACCHUS_HIP_CHECK(hipEventRecord(ev[1], s));
+		for (size_t i = 0; i < n_ops; i++) {
 			LState &l = st[i];
 			const int64_t n = l.D * l.B;
-			hipLaunchKernelGGL(to_double_cols, dim3(blocks_of(n)), dim3(256), 0, s, bt->d_y[i], l.D,
+			hipStream_t x = s;
+			if (fork) {
+				BACCHUS_HIP_CHECK(hipStreamWaitEvent(os[i], ev[1], 0));
+				x = os[i];
+			}
+			launch_batch_out(bt, i, x);
+			hipLaunchKernelGGL(to_double_cols, dim3(blocks_of(n)), dim3(256), 0, x, bt->d_y[i], l.D,
 					   l.B, bt->Bp[i], l.w);
-			project(l, 0);
-			project(l, 1);
-			norm_sq(l, l.w);
-			hipLaunchKernelGGL(finish_iteration, dim3(blocks_of(n)), dim3(256), 0, s, l.Q, l.w,
+			project(l, 0, x);
+			project(l, 1, x);
+			norm_sq(l, l.w, x);
+			hipLaunchKernelGGL(finish_iteration, dim3(blocks_of(n)), dim3(256), 0, x, l.Q, l.w,
 					   l.ss, l.beta, n, l.B, l.k);
-			hipLaunchKernelGGL(advance, dim3(1), dim3(64), 0, s, l.k);
+			hipLaunchKernelGGL(advance, dim3(1), dim3(64), 0, x, l.k);
+			if (fork) {
+				BACCHUS_HIP_CHECK(hipEventRecord(ev[2 + n_ops + i], x));
+				BACCHUS_HIP_CHECK(hipStreamWaitEvent(s, ev[2 + n_ops + i], 0));
+			}
 		}
 		BACCHUS_HIP_CHECK(hipGetLastError());
 		return 0;
@@ -1971,11 +2024,22 @@ extern "C" int bacchus_hip_batch_lanczos(bacchus_hip_batch_t *bt,
 		else
 			hipLaunchKernelGGL(random_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, l.Q, (int64_t)n,
 					   set->seed ^ (0x5bd1e995ULL * (uint64_t)(i + 1)));
-		L.norm_sq(l, l.Q);
+		L.norm_sq(l, l.Q, L.s);
 		hipLaunchKernelGGL(normalize_cols, dim3(blocks_of(n)), dim3(256), 0, L.s, l.Q, l.ss,
 				   (int64_t)n, l.B);
 	}
 	BACCHUS_HIP_CHECK(hipGetLastError());
+	// BACCHUS_HIP_LANCZOS_STREAMS=0 keeps every operator's vector work on
+	// one stream, one operator after the other
+	const char *fenv = getenv("BACCHUS_HIP_LANCZOS_STREAMS");
+	if (nops > 1 && !(fenv && atoi(fenv) == 0)) {
+		L.ev.resize(2 + 2 * (size_t)nops, nullptr);
+		for (auto &e : L.ev)
+			BACCHUS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		L.os.resize(nops, nullptr);
+		for (auto &x : L.os)
+			BACCHUS_HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+	}
 	hipGraphExec_t exec = nullptr;
 	hipGraph_t graph = nullptr;
 	if (set->use_graph) {

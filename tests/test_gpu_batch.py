@@ -214,6 +214,43 @@ def test_batched_lanczos_uneven_batch_converges():
             o.close()
 
 
+def test_batched_lanczos_operator_streams_keep_results():
+    """Three operators: their vector work on streams of their own (the
+    default) or one after the other on one stream
+    (BACCHUS_HIP_LANCZOS_STREAMS=0), captured or eager, gives the same
+    eigenvalues to the FP32 matvec's rounding."""
+    import os
+    print("case batch lanczos operator streams", flush=True)
+    inps = [build(SYSTEMS["he4_nmax4"]), build(SYSTEMS["he4_nmax6"]), build(SMALL_3B)]
+    counts = [3, 2, 2]
+    ops = [bacchus.Operator.from_inputs(i, precision=bacchus.FP32) for i in inps]
+    saved = os.environ.get("BACCHUS_HIP_LANCZOS_STREAMS")
+    runs = {}
+    try:
+        with bacchus.Batch(ops, counts) as bt:
+            for k, B in enumerate(counts):
+                bt.set_scales(k, scales_for(B, 40 + k))
+            for flag in ("1", "0"):
+                os.environ["BACCHUS_HIP_LANCZOS_STREAMS"] = flag
+                for graph in (True, False):
+                    ev, iters, _ = bt.lanczos(80, num_eigenvalues=3, use_graph=graph, seed=6)
+                    assert iters == 80
+                    runs[(flag, graph)] = ev
+    finally:
+        if saved is None:
+            os.environ.pop("BACCHUS_HIP_LANCZOS_STREAMS", None)
+        else:
+            os.environ["BACCHUS_HIP_LANCZOS_STREAMS"] = saved
+        for o in ops:
+            o.close()
+    ref = runs[("0", True)]
+    for key, ev in runs.items():
+        for k in range(len(counts)):
+            err = np.abs(ev[k] - ref[k]) / np.abs(ref[k])
+            print(key, k, err.max(), flush=True)
+            assert err.max() <= 1e-6
+
+
 def test_autotune_keeps_results():
     print("case autotune", flush=True)
     torch = torch_dev()
